@@ -1,0 +1,1047 @@
+// C-ABI entry points: context set-up, the matvec (MYMATMUL / HAMMULT),
+// Hermitian dots (MYDOTSUB) and the GPLMR / harmonic-Ritz block eigensolver
+// (GPLMR_EIG, gplmr.F90:9-386) driven from the host over device kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dex_internal.h"
+
+using dex::ColDesc;
+using dex::Dbl2;
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string &msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+  return fail(DEX_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define DEX_HIP(call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+  } while (0)
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr int kMaxDesc = 512;
+constexpr int kMaxSmall = 96 * 64;
+
+}  // namespace
+
+struct dex_ctx {
+  dex_params p{};
+  int nn = 0;
+  int64_t m = 0, d = 0;
+  std::vector<double> t_host, tinv_host, vn_host;
+  double *d_tinv = nullptr, *d_vn = nullptr;
+  dex::MatvecCoef coef{};
+  hipStream_t stream = nullptr;
+  bool fast64 = false, fast32 = false;
+  void *d_u = nullptr;  // matvec intermediate, D complex128 (also used as D complex64)
+  void *d_xs = nullptr, *d_ys = nullptr;  // complex64 staging for FP32 matvecs
+  ColDesc *d_desc = nullptr;
+  Dbl2 *d_small = nullptr;
+  Dbl2 *d_small2 = nullptr;
+  Dbl2 *d_gram_out = nullptr;
+  void *d_partial = nullptr;
+  size_t partial_bytes = 0;
+  double *d_dots = nullptr;
+  bool timing = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double t_pass1 = -1.0, t_pass2 = -1.0;
+};
+
+namespace {
+
+int ensure_u(dex_ctx *c) {
+  if (c->d_u) return 0;
+  DEX_HIP(hipMalloc(&c->d_u, size_t(c->d) * sizeof(Dbl2)));
+  return 0;
+}
+
+int ensure_single_staging(dex_ctx *c) {
+  if (c->d_xs) return 0;
+  DEX_HIP(hipMalloc(&c->d_xs, size_t(c->d) * 8));
+  DEX_HIP(hipMalloc(&c->d_ys, size_t(c->d) * 8));
+  return 0;
+}
+
+// One application of H in the requested arithmetic; x, y complex128 on device.
+int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single) {
+  if (int st = ensure_u(c)) return st;
+  hipStream_t s = c->stream;
+  if (single) {
+    if (!c->fast32) return fail(DEX_EUNSUPPORTED, "no FP32 matvec for this grid");
+    if (int st = ensure_single_staging(c)) return st;
+    DEX_HIP(dex::launch_to_single(x, c->d_xs, c->d, s));
+    if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
+    DEX_HIP(dex::launch_pass1(c->nn, true, c->d_xs, c->d_u, s));
+    if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
+    DEX_HIP(dex::launch_pass2(c->nn, true, c->d_xs, c->d_u, c->d_ys, c->d_vn, c->d_tinv, c->coef, s));
+    if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
+    DEX_HIP(dex::launch_to_double(c->d_ys, y, c->d, s));
+    return 0;
+  }
+  if (c->fast64) {
+    if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
+    DEX_HIP(dex::launch_pass1(c->nn, false, x, c->d_u, s));
+    if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
+    DEX_HIP(dex::launch_pass2(c->nn, false, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, s));
+    if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
+    return 0;
+  }
+  DEX_HIP(dex::launch_matvec_generic(c->nn, x, y, c->d_vn, c->d_tinv, c->coef, s));
+  return 0;
+}
+
+// ---- block helpers on one context ------------------------------------------
+struct Block {
+  std::vector<Dbl2 *> cols;
+  int size() const { return int(cols.size()); }
+};
+
+ColDesc plain(const Dbl2 *a) { return ColDesc{a, nullptr, 0.0, 0.0}; }
+
+int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
+  if (int(ds.size()) > kMaxDesc) return fail(DEX_EINVAL, "too many block columns");
+  DEX_HIP(hipMemcpyAsync(c->d_desc, ds.data(), ds.size() * sizeof(ColDesc), hipMemcpyHostToDevice,
+                         c->stream));
+  return 0;
+}
+
+// G = L^H R (kl x kr, column-major) on the host.
+int gram(dex_ctx *c, const std::vector<ColDesc> &l, const std::vector<ColDesc> &r,
+         std::vector<cd> &g) {
+  const int kl = int(l.size()), kr = int(r.size());
+  g.assign(size_t(kl) * kr, 0.0);
+  if (kl == 0 || kr == 0) return 0;
+  std::vector<ColDesc> ds(l);
+  ds.insert(ds.end(), r.begin(), r.end());
+  if (int st = upload_descs(c, ds)) return st;
+  DEX_HIP(dex::launch_gram(c->d_desc, kl, kr, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
+                           c->d_gram_out, c->stream));
+  DEX_HIP(hipMemcpyAsync(g.data(), c->d_gram_out, g.size() * sizeof(Dbl2), hipMemcpyDeviceToHost,
+                         c->stream));
+  DEX_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// out_j = sum_i in_i C(i, j) (+ beta_j out_j), row-local, in-place safe.
+int combine(dex_ctx *c, const std::vector<ColDesc> &in, const std::vector<Dbl2 *> &out,
+            const std::vector<cd> &cm, const std::vector<cd> *beta = nullptr) {
+  const int nin = int(in.size()), nout = int(out.size());
+  if (nout == 0) return 0;
+  if (nin * nout > kMaxSmall) return fail(DEX_EINVAL, "combine matrix too large");
+  std::vector<ColDesc> ds(in);
+  for (Dbl2 *o : out) ds.push_back(plain(o));
+  if (int st = upload_descs(c, ds)) return st;
+  DEX_HIP(hipMemcpyAsync(c->d_small, cm.data(), cm.size() * sizeof(Dbl2), hipMemcpyHostToDevice,
+                         c->stream));
+  if (beta) {
+    DEX_HIP(hipMemcpyAsync(c->d_small2, beta->data(), beta->size() * sizeof(Dbl2),
+                           hipMemcpyHostToDevice, c->stream));
+  }
+  DEX_HIP(dex::launch_combine(c->d_desc, nin, nout, c->d_small, beta ? c->d_small2 : nullptr, c->d,
+                              c->stream));
+  return 0;
+}
+
+// Per column j: aa_j = |a_j|^2, ab_j = a_j^H b_j, bb_j = |b_j|^2.
+int coldots(dex_ctx *c, const std::vector<ColDesc> &a, const std::vector<ColDesc> &b,
+            std::vector<double> &aa, std::vector<cd> &ab, std::vector<double> &bb) {
+  const int k = int(a.size());
+  std::vector<ColDesc> ds(a);
+  if (b.empty()) {
+    for (int j = 0; j < k; ++j) ds.push_back(ColDesc{nullptr, nullptr, 0, 0});
+  } else {
+    ds.insert(ds.end(), b.begin(), b.end());
+  }
+  if (int st = upload_descs(c, ds)) return st;
+  DEX_HIP(dex::launch_coldots(c->d_desc, k, c->d, reinterpret_cast<double *>(c->d_partial), c->d_dots,
+                              c->stream));
+  std::vector<double> h(size_t(k) * 4);
+  DEX_HIP(hipMemcpyAsync(h.data(), c->d_dots, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DEX_HIP(hipStreamSynchronize(c->stream));
+  aa.resize(k);
+  ab.resize(k);
+  bb.resize(k);
+  for (int j = 0; j < k; ++j) {
+    aa[j] = h[4 * j];
+    ab[j] = cd(h[4 * j + 1], h[4 * j + 2]);
+    bb[j] = h[4 * j + 3];
+  }
+  return 0;
+}
+
+std::vector<ColDesc> descs_of(const Block &b) {
+  std::vector<ColDesc> v;
+  for (Dbl2 *p : b.cols) v.push_back(plain(p));
+  return v;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+void dex_params_default(dex_params *p) {
+  if (!p) return;
+  p->n = 2;
+  p->delta = 0.5;
+  p->theta = 0.0;
+  p->charge = 2;
+  p->nsmall = 22;
+  p->tfactor = 4;
+  p->precision = DEX_FP64;
+  p->device = 0;
+}
+
+void dex_solve_opts_default(dex_solve_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->fp64_restarts = 2;
+  o->fp32_tol = 1e-5;
+}
+
+const char *dex_status_string(int status) {
+  switch (status) {
+    case DEX_OK: return "ok";
+    case DEX_ESIZE: return "size mismatch";
+    case DEX_ECHOLESKY: return "Cholesky failed";
+    case DEX_EGEVP: return "projected eigenvalue problem failed";
+    case DEX_ENOTCONVERGED: return "not converged";
+    case DEX_EHIP: return "HIP runtime error";
+    case DEX_EOOM: return "device allocation failed";
+    case DEX_EINVAL: return "invalid argument";
+    case DEX_EADMISSION: return "does not fit the HBM cap";
+    case DEX_EUNSUPPORTED: return "unsupported grid/precision";
+    case DEX_EZEROBLOCK: return "all-zero block in cholQR";
+    default: return "unknown status";
+  }
+}
+
+const char *dex_last_error(void) { return g_last_error.c_str(); }
+
+int dex_ctx_create(const dex_params *p, dex_ctx **out) {
+  if (!p || !out) return fail(DEX_EINVAL, "null argument");
+  *out = nullptr;
+  if (p->n < 1 || !(p->delta > 0) || p->nsmall < 1 || p->tfactor < 1 ||
+      (p->precision != DEX_FP64 && p->precision != DEX_FP32))
+    return fail(DEX_EINVAL, "invalid parameters");
+  const int nn = 2 * p->n + 1;
+  if (!dex::generic_matvec_supported(nn)) return fail(DEX_EUNSUPPORTED, "NN not built (odd 3..25)");
+  if (p->nsmall * p->tfactor < nn) return fail(DEX_EINVAL, "NSMALL*TFACTOR < NN (tooth.f90:38)");
+  DEX_HIP(hipSetDevice(p->device));
+  auto *c = new dex_ctx();
+  c->p = *p;
+  c->nn = nn;
+  c->m = int64_t(nn) * nn * nn;
+  c->d = c->m * c->m;
+  c->t_host.resize(size_t(nn) * nn);
+  dex::kematrix(nn, p->delta, c->t_host.data());
+  if (dex::getinverse(nn, p->delta, p->nsmall, p->tfactor, c->tinv_host) != 0) {
+    delete c;
+    return fail(DEX_EINVAL, "GETINVERSE failed");
+  }
+  // A Tinv(r_p), A = -Z/delta^3 (hammod.F90:554,571), nucleus at the origin
+  const int L = 2 * nn + 1;
+  const int n = p->n;
+  c->vn_host.resize(size_t(c->m));
+  const double a = -double(p->charge) / (p->delta * p->delta * p->delta);
+  for (int k = 0; k < nn; ++k)
+    for (int j = 0; j < nn; ++j)
+      for (int i = 0; i < nn; ++i) {
+        const int ti = (i - n + nn) + L * ((j - n + nn) + L * (k - n + nn));
+        c->vn_host[i + nn * (j + nn * k)] = a * c->tinv_host[ti];
+      }
+  // T / SCALE^2 and V / SCALE with SCALE = e^{i theta} (initmod.F90:48,207,224)
+  const cd scale(std::cos(p->theta), std::sin(p->theta));
+  const cd s2 = 1.0 / (scale * scale) / (p->delta * p->delta);
+  const cd c1 = 1.0 / scale;
+  c->coef.s2_re = s2.real();
+  c->coef.s2_im = s2.imag();
+  c->coef.c1_re = c1.real();
+  c->coef.c1_im = c1.imag();
+  c->coef.inv_d3 = 1.0 / (p->delta * p->delta * p->delta);
+  c->fast64 = dex::fast_matvec_supported(nn, false);
+  c->fast32 = dex::fast_matvec_supported(nn, true);
+  // DEX_MATVEC=generic selects the six-pass FP64 kernels even where the LDS
+  // two-pass kernels exist (used to cross-check the two implementations).
+  if (const char *mv = std::getenv("DEX_MATVEC"))
+    if (std::strcmp(mv, "generic") == 0) c->fast64 = false;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->d_tinv, c->tinv_host.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->d_vn, c->vn_host.size() * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemcpy(c->d_tinv, c->tinv_host.data(), c->tinv_host.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(c->d_vn, c->vn_host.data(), c->vn_host.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->d_desc, kMaxDesc * sizeof(ColDesc));
+  if (e == hipSuccess) e = hipMalloc(&c->d_small, kMaxSmall * sizeof(Dbl2));
+  if (e == hipSuccess) e = hipMalloc(&c->d_small2, kMaxSmall * sizeof(Dbl2));
+  if (e == hipSuccess) e = hipMalloc(&c->d_gram_out, 192 * 192 * sizeof(Dbl2));
+  c->partial_bytes = std::max(dex::gram_partial_elems(64, 128) * sizeof(Dbl2),
+                              dex::coldots_partial_elems(64) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->d_partial, c->partial_bytes);
+  if (e == hipSuccess) e = hipMalloc(&c->d_dots, 64 * 4 * sizeof(double));
+  for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+  if (e != hipSuccess) {
+    dex_ctx_destroy(c);
+    return hip_fail(e, "dex_ctx_create");
+  }
+  *out = c;
+  return DEX_OK;
+}
+
+int dex_ctx_destroy(dex_ctx *c) {
+  if (!c) return DEX_OK;
+  (void)hipSetDevice(c->p.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *ptr : {(void *)c->d_tinv, (void *)c->d_vn, c->d_u, c->d_xs, c->d_ys, (void *)c->d_desc,
+                    (void *)c->d_small, (void *)c->d_small2, (void *)c->d_gram_out, c->d_partial,
+                    (void *)c->d_dots})
+    if (ptr) (void)hipFree(ptr);
+  for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return DEX_OK;
+}
+
+int dex_ctx_info(const dex_ctx *c, dex_info *info) {
+  if (!c || !info) return fail(DEX_EINVAL, "null argument");
+  info->n = c->p.n;
+  info->nn = c->nn;
+  info->m = c->m;
+  info->d = c->d;
+  info->delta = c->p.delta;
+  info->theta = c->p.theta;
+  info->precision = c->p.precision;
+  info->fast_path = c->p.precision == DEX_FP32 ? int(c->fast32) : int(c->fast64);
+  return DEX_OK;
+}
+
+int dex_get_kematrix(const dex_ctx *c, double *t) {
+  if (!c || !t) return fail(DEX_EINVAL, "null argument");
+  std::memcpy(t, c->t_host.data(), c->t_host.size() * sizeof(double));
+  return DEX_OK;
+}
+
+int dex_get_tinv(const dex_ctx *c, double *tinv) {
+  if (!c || !tinv) return fail(DEX_EINVAL, "null argument");
+  std::memcpy(tinv, c->tinv_host.data(), c->tinv_host.size() * sizeof(double));
+  return DEX_OK;
+}
+
+int dex_set_kernel_timing(dex_ctx *c, int32_t enable) {
+  if (!c) return fail(DEX_EINVAL, "null argument");
+  c->timing = enable != 0;
+  return DEX_OK;
+}
+
+int dex_get_kernel_timing(const dex_ctx *c, double *p1, double *p2) {
+  if (!c || !p1 || !p2) return fail(DEX_EINVAL, "null argument");
+  *p1 = c->t_pass1;
+  *p2 = c->t_pass2;
+  return DEX_OK;
+}
+
+int dex_matvec_device(dex_ctx *c, const void *in, void *out, int32_t howmany, int64_t checksize,
+                      int32_t single) {
+  if (!c || !in || !out) return fail(DEX_EINVAL, "null argument");
+  if (checksize != c->d) return fail(DEX_ESIZE, "SIZE ERROR: checksize != M*MBLK");
+  if (howmany < 0) return fail(DEX_EINVAL, "howmany < 0");
+  DEX_HIP(hipSetDevice(c->p.device));
+  const bool sp = single != 0;
+  if (sp && c->p.precision != DEX_FP32) return fail(DEX_EINVAL, "complex64 I/O needs an FP32 context");
+  if (sp && !c->fast32) return fail(DEX_EUNSUPPORTED, "no FP32 matvec for this grid");
+  if (int st = ensure_u(c)) return st;
+  for (int j = 0; j < howmany; ++j) {
+    if (sp) {
+      const char *xi = static_cast<const char *>(in) + size_t(j) * c->d * 8;
+      char *yo = static_cast<char *>(out) + size_t(j) * c->d * 8;
+      if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], c->stream));
+      DEX_HIP(dex::launch_pass1(c->nn, true, xi, c->d_u, c->stream));
+      if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], c->stream));
+      DEX_HIP(dex::launch_pass2(c->nn, true, xi, c->d_u, yo, c->d_vn, c->d_tinv, c->coef, c->stream));
+      if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], c->stream));
+    } else {
+      const Dbl2 *xi = static_cast<const Dbl2 *>(in) + size_t(j) * c->d;
+      Dbl2 *yo = static_cast<Dbl2 *>(out) + size_t(j) * c->d;
+      if (int st = apply_h(c, xi, yo, c->p.precision == DEX_FP32)) return st;
+    }
+  }
+  if (c->timing && howmany > 0) {
+    DEX_HIP(hipEventSynchronize(c->ev[2]));
+    float a = 0.f, b = 0.f;
+    DEX_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    DEX_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    c->t_pass1 = a;
+    c->t_pass2 = b;
+  }
+  return DEX_OK;
+}
+
+int dex_matvec(dex_ctx *c, const double *in, double *out, int32_t howmany, int64_t checksize) {
+  if (!c || !in || !out) return fail(DEX_EINVAL, "null argument");
+  if (checksize != c->d) return fail(DEX_ESIZE, "SIZE ERROR: checksize != M*MBLK");
+  DEX_HIP(hipSetDevice(c->p.device));
+  Dbl2 *dx = nullptr, *dy = nullptr;
+  const size_t bytes = size_t(c->d) * sizeof(Dbl2);
+  DEX_HIP(hipMalloc(&dx, bytes));
+  hipError_t e = hipMalloc(&dy, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(dx);
+    return hip_fail(e, "hipMalloc");
+  }
+  int st = DEX_OK;
+  for (int j = 0; j < howmany && st == DEX_OK; ++j) {
+    e = hipMemcpyAsync(dx, in + size_t(j) * c->d * 2, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+      st = hip_fail(e, "copy in");
+      break;
+    }
+    st = dex_matvec_device(c, dx, dy, 1, checksize, 0);
+    if (st) break;
+    e = hipMemcpyAsync(out + size_t(j) * c->d * 2, dy, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) st = hip_fail(e, "copy out");
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return st;
+}
+
+int dex_dots_device(dex_ctx *c, const void *bras, const void *kets, int64_t size, int32_t nn,
+                    int32_t mm, double *out) {
+  if (!c || !bras || !kets || !out) return fail(DEX_EINVAL, "null argument");
+  if (size != c->d) return fail(DEX_ESIZE, "SIZE ERROR: size != M*MBLK");
+  if (nn < 1 || mm < 1 || nn + mm > 192) return fail(DEX_EINVAL, "nn, mm out of range");
+  DEX_HIP(hipSetDevice(c->p.device));
+  std::vector<ColDesc> l, r;
+  for (int i = 0; i < nn; ++i) l.push_back(plain(static_cast<const Dbl2 *>(bras) + size_t(i) * size));
+  for (int j = 0; j < mm; ++j) r.push_back(plain(static_cast<const Dbl2 *>(kets) + size_t(j) * size));
+  std::vector<cd> g;
+  if (int st = gram(c, l, r, g)) return st;
+  std::memcpy(out, g.data(), g.size() * sizeof(cd));
+  return DEX_OK;
+}
+
+int dex_dots(dex_ctx *c, const double *bras, const double *kets, int64_t size, int32_t nn,
+             int32_t mm, double *out) {
+  if (!c || !bras || !kets || !out) return fail(DEX_EINVAL, "null argument");
+  if (size != c->d) return fail(DEX_ESIZE, "SIZE ERROR: size != M*MBLK");
+  DEX_HIP(hipSetDevice(c->p.device));
+  void *db = nullptr, *dk = nullptr;
+  DEX_HIP(hipMalloc(&db, size_t(nn) * size * 16));
+  hipError_t e = hipMalloc(&dk, size_t(mm) * size * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(db);
+    return hip_fail(e, "hipMalloc");
+  }
+  int st = DEX_OK;
+  e = hipMemcpy(db, bras, size_t(nn) * size * 16, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dk, kets, size_t(mm) * size * 16, hipMemcpyHostToDevice);
+  if (e != hipSuccess)
+    st = hip_fail(e, "copy in");
+  else
+    st = dex_dots_device(c, db, dk, size, nn, mm, out);
+  (void)hipFree(db);
+  (void)hipFree(dk);
+  return st;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// GPLMR
+namespace {
+
+struct Solver {
+  dex_ctx *c;
+  int nev, ns;
+  bool strict;
+  bool single;  // FP32 matvec
+  int fallbacks = 0;
+  int64_t matvecs = 0;
+  double t_matvec = 0, t_gram = 0, t_update = 0, t_small = 0;
+  std::vector<Dbl2 *> pool;
+
+  int matblock(const Block &in, const Block &out) {
+    const double t0 = now_s();
+    for (int j = 0; j < in.size(); ++j) {
+      if (int st = apply_h(c, in.cols[j], out.cols[j], single)) return st;
+      ++matvecs;
+    }
+    DEX_HIP(hipStreamSynchronize(c->stream));
+    t_matvec += now_s() - t0;
+    return 0;
+  }
+
+  // cholQR (lamod.f90:128-180) with the rank-revealing fallback.  Returns in
+  // `t` (k x k', column-major) the transform with xq = x t; k' < k only when
+  // `drop` removed dependent directions.
+  int orth_transform(const Block &x, bool drop, std::vector<cd> &t, int &kout) {
+    const int k = x.size();
+    std::vector<cd> g;
+    double t0 = now_s();
+    if (int st = gram(c, descs_of(x), descs_of(x), g)) return st;
+    t_gram += now_s() - t0;
+    t0 = now_s();
+    double asum = 0.0;
+    for (const cd &v : g) asum += std::abs(v);
+    if (asum == 0.0) {
+      if (drop) {
+        kout = 0;
+        t.clear();
+        return 0;
+      }
+      return fail(DEX_EZEROBLOCK, "all-zero block in cholQR");
+    }
+    std::vector<double> dg(k);
+    bool ok = true;
+    for (int i = 0; i < k; ++i) {
+      dg[i] = std::sqrt(std::max(g[i + size_t(k) * i].real(), 0.0));
+      if (!(dg[i] > 0)) ok = false;
+    }
+    std::vector<cd> r = g;
+    if (ok && dex::zpotrf_upper(k, r.data(), k) == 0) {
+      for (int i = 0; i < k && ok; ++i) {
+        const double piv = r[i + size_t(k) * i].real() / dg[i];
+        if (!(piv * piv > 1e-12)) ok = false;
+      }
+    } else {
+      ok = false;
+    }
+    if (ok) {
+      dex::ztrtri_upper(k, r.data(), k);
+      t = r;
+      kout = k;
+      t_small += now_s() - t0;
+      return 0;
+    }
+    if (strict) return fail(DEX_ECHOLESKY, "Cholesky failed");
+    ++fallbacks;
+    // SVQB on the unit-diagonal Gram
+    std::vector<cd> gs(size_t(k) * k);
+    std::vector<double> dinv(k);
+    for (int i = 0; i < k; ++i) dinv[i] = dg[i] > 0 ? 1.0 / dg[i] : 0.0;
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < k; ++i) gs[i + size_t(k) * j] = dinv[i] * g[i + size_t(k) * j] * dinv[j];
+    std::vector<double> w;
+    std::vector<cd> v;
+    dex::zheev_jacobi(k, gs.data(), k, w, v);
+    const double lmax = *std::max_element(w.begin(), w.end());
+    std::vector<int> keep;
+    for (int i = 0; i < k; ++i)
+      if (!drop || w[i] > 1e-12 * lmax) keep.push_back(i);
+    kout = int(keep.size());
+    t.assign(size_t(k) * kout, 0.0);
+    for (int jj = 0; jj < kout; ++jj) {
+      const int j = keep[jj];
+      const double lam = std::max(w[j], 1e-12 * lmax);
+      for (int i = 0; i < k; ++i) t[i + size_t(k) * jj] = dinv[i] * v[i + size_t(k) * j] / std::sqrt(lam);
+    }
+    t_small += now_s() - t0;
+    return 0;
+  }
+
+  // Orthonormalise block x in place (and its image ax if given) with a second
+  // cholQR pass after a fallback.  Returns the new column count.
+  int orth(Block &x, Block *ax, bool drop) {
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<cd> t;
+      int kout = 0;
+      const int k = x.size();
+      const int before = fallbacks;
+      if (int st = orth_transform(x, drop, t, kout)) return -st;
+      const double t0 = now_s();
+      if (kout > 0) {
+        std::vector<ColDesc> in = descs_of(x);
+        std::vector<Dbl2 *> out(x.cols.begin(), x.cols.begin() + kout);
+        if (int st = combine(c, in, out, t)) return -st;
+        if (ax) {
+          std::vector<ColDesc> ain = descs_of(*ax);
+          std::vector<Dbl2 *> aout(ax->cols.begin(), ax->cols.begin() + kout);
+          if (int st = combine(c, ain, aout, t)) return -st;
+        }
+      }
+      t_update += now_s() - t0;
+      for (int j = kout; j < k; ++j) pool.push_back(x.cols[j]);
+      x.cols.resize(kout);
+      if (ax) {
+        for (int j = kout; j < k; ++j) pool.push_back(ax->cols[j]);
+        ax->cols.resize(kout);
+      }
+      if (fallbacks == before || kout == 0) break;  // plain cholQR path: one pass
+    }
+    return 0;
+  }
+
+  // b <- b - q (q^H b); the image (ab <- ab - aq (q^H b)) when given.
+  int project_out(const Block &q, const Block *aq, Block &b, Block *ab) {
+    if (q.size() == 0 || b.size() == 0) return 0;
+    std::vector<cd> g;
+    double t0 = now_s();
+    if (int st = gram(c, descs_of(q), descs_of(b), g)) return st;
+    t_gram += now_s() - t0;
+    t0 = now_s();
+    const int kq = q.size(), kb = b.size();
+    // C = [I; -G]
+    std::vector<cd> cm(size_t(kb + kq) * kb, 0.0);
+    for (int j = 0; j < kb; ++j) {
+      cm[j + size_t(kb + kq) * j] = 1.0;
+      for (int i = 0; i < kq; ++i) cm[kb + i + size_t(kb + kq) * j] = -g[i + size_t(kq) * j];
+    }
+    std::vector<ColDesc> in = descs_of(b);
+    for (Dbl2 *p : q.cols) in.push_back(plain(p));
+    if (int st = combine(c, in, b.cols, cm)) return st;
+    if (ab) {
+      std::vector<ColDesc> ain = descs_of(*ab);
+      for (Dbl2 *p : aq->cols) ain.push_back(plain(p));
+      if (int st = combine(c, ain, ab->cols, cm)) return st;
+    }
+    t_update += now_s() - t0;
+    return 0;
+  }
+
+  std::vector<Dbl2 *> *store = nullptr;  // every buffer ever handed out (freed by the caller)
+
+  // A free work vector: from the pool, else a fresh allocation.  nullptr on OOM.
+  Dbl2 *take() {
+    if (!pool.empty()) {
+      Dbl2 *p = pool.back();
+      pool.pop_back();
+      return p;
+    }
+    Dbl2 *p = nullptr;
+    if (hipMalloc(&p, size_t(c->d) * sizeof(Dbl2)) != hipSuccess) return nullptr;
+    store->push_back(p);
+    return p;
+  }
+
+  // Appends k work vectors to b, or returns DEX_EOOM.
+  int grow(Block &b, int k) {
+    for (int j = 0; j < k; ++j) {
+      Dbl2 *p = take();
+      if (!p) return fail(DEX_EOOM, "device allocation for GPLMR work vectors failed");
+      b.cols.push_back(p);
+    }
+    return 0;
+  }
+};
+
+int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int maxit, double tol,
+              int inner, int outmodulus, const dex_solve_opts &o, dex_result *res, bool single,
+              std::vector<Dbl2 *> &store) {
+  Solver S{c, nev, ns, o.strict_cholesky != 0, single};
+  // every buffer in `store` is free at entry
+  const size_t vbytes = size_t(c->d) * sizeof(Dbl2);
+  // columns: X in xdev; the rest from `store` (pool of (5 + 2 ns) nev vectors)
+  Block X, AX, W, AW, P, AP;
+  for (int j = 0; j < nev; ++j) X.cols.push_back(xdev + size_t(j) * c->d);
+  S.pool = store;
+  S.store = &store;
+  if (int st = S.grow(AX, nev)) return st;
+  if (int st = S.grow(W, nev)) return st;
+  (void)vbytes;
+  const double t_start = now_s();
+  // NORMALIZE(X) (gplmr.F90:65)
+  {
+    std::vector<double> aa, bb;
+    std::vector<cd> ab;
+    if (int st = coldots(c, descs_of(X), {}, aa, ab, bb)) return st;
+    std::vector<cd> dm(size_t(nev) * nev, 0.0);
+    for (int j = 0; j < nev; ++j) {
+      if (!(aa[j] > 0)) return fail(DEX_EZEROBLOCK, "zero start vector");
+      dm[j + size_t(nev) * j] = 1.0 / std::sqrt(aa[j]);
+    }
+    if (int st = combine(c, descs_of(X), X.cols, dm)) return st;
+  }
+  if (int st = S.matblock(X, AX)) return st;  // :68
+  std::vector<cd> e(nev);
+  std::vector<double> resn(nev);
+  auto residual = [&]() -> int {
+    // E = diag(X^H AX) of unit columns and W = AX - X E, ||W_j|| (:324-341):
+    // X and AX are first scaled by 1/||X_j||.
+    std::vector<double> aa, bb;
+    std::vector<cd> ab;
+    double t0 = now_s();
+    if (int st = coldots(c, descs_of(X), descs_of(AX), aa, ab, bb)) return st;
+    S.t_gram += now_s() - t0;
+    t0 = now_s();
+    std::vector<cd> cm(size_t(2 * nev) * 3 * nev, 0.0);
+    const int ld = 2 * nev;
+    for (int j = 0; j < nev; ++j) {
+      const double n = std::sqrt(aa[j]);
+      const double dinv = 1.0 / n;
+      e[j] = ab[j] / aa[j];
+      cm[j + size_t(ld) * j] = dinv;                      // X_j / n
+      cm[nev + j + size_t(ld) * (nev + j)] = dinv;        // AX_j / n
+      cm[nev + j + size_t(ld) * (2 * nev + j)] = dinv;    // W_j = AX_j/n - X_j E_j / n
+      cm[j + size_t(ld) * (2 * nev + j)] = -e[j] * dinv;
+    }
+    std::vector<ColDesc> in = descs_of(X);
+    for (Dbl2 *p : AX.cols) in.push_back(plain(p));
+    std::vector<Dbl2 *> out = X.cols;
+    out.insert(out.end(), AX.cols.begin(), AX.cols.end());
+    out.insert(out.end(), W.cols.begin(), W.cols.end());
+    if (int st = combine(c, in, out, cm)) return st;
+    S.t_update += now_s() - t0;
+    t0 = now_s();
+    if (int st = coldots(c, descs_of(W), {}, aa, ab, bb)) return st;
+    S.t_gram += now_s() - t0;
+    for (int j = 0; j < nev; ++j) resn[j] = std::sqrt(aa[j]);
+    return 0;
+  };
+  // First E and W: X is already unit, the scaling is 1.
+  if (int st = residual()) return st;
+  double maxres = *std::max_element(resn.begin(), resn.end());
+  int iter = 1;
+  Block Sblk, ASblk;
+  while (iter <= maxit && maxres > tol) {
+    if (o.refresh_every > 0 && iter > 1 && (iter - 1) % o.refresh_every == 0) {
+      if (int st = S.matblock(X, AX)) return st;
+    }
+    // CHOLQR(X); AX <- AX R^{-1}  (:88-90)
+    if (int st = S.orth(X, &AX, false)) return -st;
+    if (X.size() != nev) return fail(DEX_ECHOLESKY, "X lost rank");
+    // PRECBLOCK with GM_MAX_INNER_ITER = 1: W <- H W (:92-99, :491-493)
+    if (inner == 1) {
+      Block HW;
+      if (int st = S.grow(HW, W.size())) return st;
+      if (int st = S.matblock(W, HW)) return st;
+      for (Dbl2 *p : W.cols) S.pool.push_back(p);
+      W = HW;
+    }
+    // W <- W - X (X^H W); CHOLQR(W)  (:101-105)
+    if (int st = S.project_out(X, nullptr, W, nullptr)) return st;
+    if (int st = S.orth(W, nullptr, true)) return -st;
+    AW.cols.clear();
+    if (int st = S.grow(AW, W.size())) return st;
+    if (int st = S.matblock(W, AW)) return st;  // :108
+    // S blocks (:112-172)
+    for (Dbl2 *p : Sblk.cols) S.pool.push_back(p);
+    for (Dbl2 *p : ASblk.cols) S.pool.push_back(p);
+    Sblk.cols.clear();
+    ASblk.cols.clear();
+    Block prev = W, aprev = AW;
+    for (int l = 1; l <= ns; ++l) {
+      Block sl;
+      const int k = prev.size();
+      if (int st = S.grow(sl, k)) return st;
+      if (k > 0) {
+        // S_l = A S_{l-1} - S_{l-1} E  (W for l = 1)
+        std::vector<cd> cm(size_t(2 * k) * k, 0.0);
+        for (int j = 0; j < k; ++j) {
+          cm[j + size_t(2 * k) * j] = 1.0;
+          cm[k + j + size_t(2 * k) * j] = -e[j];
+        }
+        std::vector<ColDesc> in = descs_of(aprev);
+        for (Dbl2 *p : prev.cols) in.push_back(plain(p));
+        if (int st = combine(c, in, sl.cols, cm)) return st;
+      }
+      if (inner == 1 && sl.size() > 0) {
+        Block hs;
+        if (int st = S.grow(hs, sl.size())) return st;
+        if (int st = S.matblock(sl, hs)) return st;
+        for (Dbl2 *p : sl.cols) S.pool.push_back(p);
+        sl = hs;
+      }
+      if (int st = S.project_out(X, nullptr, sl, nullptr)) return st;
+      if (int st = S.project_out(W, nullptr, sl, nullptr)) return st;
+      if (int st = S.project_out(Sblk, nullptr, sl, nullptr)) return st;
+      if (int st = S.orth(sl, nullptr, true)) return -st;
+      Block asl;
+      if (int st = S.grow(asl, sl.size())) return st;
+      if (int st = S.matblock(sl, asl)) return st;
+      Sblk.cols.insert(Sblk.cols.end(), sl.cols.begin(), sl.cols.end());
+      ASblk.cols.insert(ASblk.cols.end(), asl.cols.begin(), asl.cols.end());
+      prev = sl;
+      aprev = asl;
+    }
+    // P against X, W, S; CHOLQR(P); AP <- AP R^{-1}  (:177-196)
+    if (iter > 1 && P.size() > 0) {
+      if (int st = S.project_out(X, &AX, P, &AP)) return st;
+      if (int st = S.project_out(W, &AW, P, &AP)) return st;
+      if (int st = S.project_out(Sblk, &ASblk, P, &AP)) return st;
+      if (int st = S.orth(P, &AP, true)) return -st;
+    }
+    // Harmonic projection K = (AQ - sigma Q)^H (AQ - sigma Q), M = (AQ - sigma Q)^H Q (:201-277)
+    Block Q, AQ;
+    for (const Block *b : {&X, &W, &Sblk, &P}) Q.cols.insert(Q.cols.end(), b->cols.begin(), b->cols.end());
+    for (const Block *b : {&AX, &AW, &ASblk, &AP})
+      AQ.cols.insert(AQ.cols.end(), b->cols.begin(), b->cols.end());
+    const int nq = Q.size();
+    if (nq > 64) return fail(DEX_EINVAL, "projected basis larger than 64");
+    std::vector<ColDesc> bl;
+    for (int j = 0; j < nq; ++j) bl.push_back(ColDesc{AQ.cols[j], Q.cols[j], sigma.real(), sigma.imag()});
+    std::vector<cd> km;
+    std::vector<ColDesc> rr = bl;
+    for (Dbl2 *p : Q.cols) rr.push_back(plain(p));
+    double t0 = now_s();
+    if (int st = gram(c, bl, rr, km)) return st;
+    S.t_gram += now_s() - t0;
+    t0 = now_s();
+    std::vector<cd> kp(size_t(nq) * nq), mp(size_t(nq) * nq);
+    for (int j = 0; j < nq; ++j)
+      for (int i = 0; i < nq; ++i) {
+        kp[i + size_t(nq) * j] = km[i + size_t(nq) * j];
+        mp[i + size_t(nq) * j] = km[i + size_t(nq) * (nq + j)];
+      }
+    std::vector<cd> coords, mu;
+    if (dex::harmonic_pencil(nq, kp.data(), mp.data(), nq, nev, coords, mu) != 0)
+      return fail(DEX_EGEVP, "SOLVE OF THE PROJECTED EIGENVALUE PROBLEM FAILED");
+    S.t_small += now_s() - t0;
+    t0 = now_s();
+    // P = W CW + S CS + P CP;  X = X CX + P  (and images) (:298-320)
+    const int nw = W.size(), nsb = Sblk.size(), np = P.size();
+    // inputs [X W S P], outputs [X, Pnew]
+    Block Pn, APn;
+    for (int j = 0; j < np && j < nev; ++j) {
+      Pn.cols.push_back(P.cols[j]);
+      APn.cols.push_back(AP.cols[j]);
+    }
+    if (int st = S.grow(Pn, nev - Pn.size())) return st;
+    if (int st = S.grow(APn, nev - APn.size())) return st;
+    std::vector<cd> cm(size_t(nq) * 2 * nev, 0.0);
+    for (int j = 0; j < nev; ++j)
+      for (int i = 0; i < nq; ++i) {
+        const cd v = coords[i + size_t(nq) * j];
+        cm[i + size_t(nq) * j] = v;                              // X_new
+        if (i >= nev) cm[i + size_t(nq) * (nev + j)] = v;        // P_new
+      }
+    {
+      std::vector<Dbl2 *> out = X.cols;
+      out.insert(out.end(), Pn.cols.begin(), Pn.cols.end());
+      if (int st = combine(c, descs_of(Q), out, cm)) return st;
+      std::vector<Dbl2 *> aout = AX.cols;
+      aout.insert(aout.end(), APn.cols.begin(), APn.cols.end());
+      if (int st = combine(c, descs_of(AQ), aout, cm)) return st;
+    }
+    (void)nw;
+    (void)nsb;
+    for (int j = nev; j < np; ++j) {
+      S.pool.push_back(P.cols[j]);
+      S.pool.push_back(AP.cols[j]);
+    }
+    P = Pn;
+    AP = APn;
+    DEX_HIP(hipStreamSynchronize(c->stream));
+    S.t_update += now_s() - t0;
+    // free W and AW columns (W is rebuilt as the residual in place of the old W storage)
+    for (Dbl2 *p : AW.cols) S.pool.push_back(p);
+    AW.cols.clear();
+    // W gets nev columns again
+    if (int st = S.grow(W, nev - W.size())) return st;
+    if (int st = residual()) return st;
+    maxres = *std::max_element(resn.begin(), resn.end());
+    if (o.verbose && (maxres <= tol || (outmodulus > 0 && iter % outmodulus == 0))) {
+      std::printf("ITERATION %5d\n", iter);
+      for (int j = 0; j < nev; ++j)
+        std::printf("EIG %4d = %20.12f%20.12f,  ERROR = %10.2E\n", j + 1, e[j].real(), e[j].imag(), resn[j]);
+      std::fflush(stdout);
+    }
+    ++iter;
+  }
+  for (int j = 0; j < nev; ++j) e_out[j] = e[j];
+  if (res) {
+    res->iterations = iter - 1;
+    res->converged = maxres <= tol;
+    res->matvecs += S.matvecs;
+    res->max_residual = maxres;
+    res->t_total += now_s() - t_start;
+    res->t_matvec += S.t_matvec;
+    res->t_gram += S.t_gram;
+    res->t_update += S.t_update;
+    res->t_smalleig += S.t_small;
+    res->fallbacks += S.fallbacks;
+  }
+  return maxres <= tol ? DEX_OK : DEX_ENOTCONVERGED;
+}
+
+}  // namespace
+
+extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns) {
+  if (!c) return -1;
+  // X plus (5 + 2 NS) NEV work vectors, the matvec intermediate and staging.
+  const int64_t vec = c->d * 16;
+  int64_t total = int64_t(6 + 2 * ns) * nev * vec + vec;
+  if (c->p.precision == DEX_FP32) total += 2 * c->d * 8;
+  return total;
+}
+
+extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,
+                                    double sigma_re, double sigma_im, int32_t nev, int32_t ns,
+                                    int32_t maxit, double tol, int32_t inner_iter, int32_t outmodulus,
+                                    const dex_solve_opts *opts, dex_result *res) {
+  if (!c || !x_dev || !e) return fail(DEX_EINVAL, "null argument");
+  if (int64_t(m) * mb != c->d) return fail(DEX_ESIZE, "SIZE ERROR: M*MB != NN^6");
+  if (nev < 1 || ns < 1 || (3 + ns) * nev > 64) return fail(DEX_EINVAL, "need NEV >= 1, NS >= 1, (3+NS)*NEV <= 64");
+  if (inner_iter != 0 && inner_iter != 1) return fail(DEX_EUNSUPPORTED, "GM_MAX_INNER_ITER must be 0 or 1");
+  DEX_HIP(hipSetDevice(c->p.device));
+  dex_solve_opts o;
+  dex_solve_opts_default(&o);
+  if (opts) o = *opts;
+  if (res) std::memset(res, 0, sizeof(*res));
+  const int nwork = (5 + 2 * ns) * nev + nev;  // pool plus headroom for the new P block
+  std::vector<Dbl2 *> store;
+  for (int j = 0; j < nwork; ++j) {
+    Dbl2 *p = nullptr;
+    hipError_t err = hipMalloc(&p, size_t(c->d) * sizeof(Dbl2));
+    if (err != hipSuccess) {
+      for (Dbl2 *q : store) (void)hipFree(q);
+      return fail(DEX_EOOM, "device allocation for GPLMR work vectors failed");
+    }
+    store.push_back(p);
+  }
+  const cd sigma(sigma_re, sigma_im);
+  std::vector<cd> ev(nev);
+  int st;
+  const bool mixed = c->p.precision == DEX_FP32;
+  // FP32 contexts: the FP32-matvec phase runs to max(tol, fp32_tol); FP64
+  // restarts from its block then reach tol.
+  const double tol1 = mixed ? std::max(tol, o.fp32_tol) : tol;
+  st = gplmr_run(c, static_cast<Dbl2 *>(x_dev), ev.data(), sigma, nev, ns, maxit, tol1, inner_iter,
+                 outmodulus, o, res, mixed, store);
+  if (mixed && (st == DEX_OK || st == DEX_ENOTCONVERGED)) {
+    for (int r = 0; r < o.fp64_restarts; ++r) {
+      dex_result sub{};
+      st = gplmr_run(c, static_cast<Dbl2 *>(x_dev), ev.data(), sigma, nev, ns, maxit, tol, inner_iter,
+                     outmodulus, o, res ? &sub : nullptr, false, store);
+      if (res) {
+        res->matvecs += sub.matvecs;
+        res->iterations += sub.iterations;
+        res->t_total += sub.t_total;
+        res->t_matvec += sub.t_matvec;
+        res->t_gram += sub.t_gram;
+        res->t_update += sub.t_update;
+        res->t_smalleig += sub.t_smalleig;
+        res->fallbacks += sub.fallbacks;
+        res->max_residual = sub.max_residual;
+        res->converged = sub.converged;
+        res->fp64_restarts_used = r + 1;
+      }
+      if (st != DEX_ENOTCONVERGED) break;
+    }
+  }
+  for (Dbl2 *q : store) (void)hipFree(q);
+  for (int j = 0; j < nev; ++j) {
+    e[2 * j] = ev[j].real();
+    e[2 * j + 1] = ev[j].imag();
+  }
+  return st;
+}
+
+extern "C" int dex_gplmr_eig(dex_ctx *c, double *x, double *e, int32_t m, int32_t mb, double sigma_re,
+                             double sigma_im, int32_t nev, int32_t ns, int32_t maxit, double tol,
+                             int32_t inner_iter, int32_t outmodulus, const dex_solve_opts *opts,
+                             dex_result *res) {
+  if (!c || !x || !e) return fail(DEX_EINVAL, "null argument");
+  if (int64_t(m) * mb != c->d) return fail(DEX_ESIZE, "SIZE ERROR: M*MB != NN^6");
+  if (nev < 1) return fail(DEX_EINVAL, "NEV < 1");
+  DEX_HIP(hipSetDevice(c->p.device));
+  const size_t bytes = size_t(c->d) * nev * sizeof(Dbl2);
+  void *dx = nullptr;
+  DEX_HIP(hipMalloc(&dx, bytes));
+  hipError_t err = hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(dx);
+    return hip_fail(err, "copy in");
+  }
+  int st = dex_gplmr_eig_device(c, dx, e, m, mb, sigma_re, sigma_im, nev, ns, maxit, tol, inner_iter,
+                                outmodulus, opts, res);
+  err = hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost);
+  (void)hipFree(dx);
+  if (err != hipSuccess && st == DEX_OK) return hip_fail(err, "copy out");
+  return st;
+}
+
+extern "C" int dex_residuals_device(dex_ctx *c, const void *x_dev, const double *e, int32_t nev,
+                                    double *res_out) {
+  if (!c || !x_dev || !e || !res_out) return fail(DEX_EINVAL, "null argument");
+  DEX_HIP(hipSetDevice(c->p.device));
+  Dbl2 *ax = nullptr;
+  DEX_HIP(hipMalloc(&ax, size_t(c->d) * sizeof(Dbl2)));
+  int st = DEX_OK;
+  for (int j = 0; j < nev && st == DEX_OK; ++j) {
+    const Dbl2 *xj = static_cast<const Dbl2 *>(x_dev) + size_t(j) * c->d;
+    st = apply_h(c, xj, ax, false);
+    if (st) break;
+    std::vector<double> aa, bb;
+    std::vector<cd> ab;
+    // r = AX - E X: coldots of (AX - E X) via the sigma descriptor
+    std::vector<ColDesc> a{ColDesc{ax, xj, e[2 * j], e[2 * j + 1]}};
+    std::vector<ColDesc> b{plain(xj)};
+    st = coldots(c, a, b, aa, ab, bb);
+    if (st) break;
+    res_out[j] = std::sqrt(aa[0]) / std::sqrt(bb[0]);
+  }
+  (void)hipFree(ax);
+  return st;
+}
+
+extern "C" int dex_fill_gaussian_device(dex_ctx *c, void *x_dev, int32_t ncols, uint64_t seed,
+                                        uint32_t stream_id, int32_t singlet) {
+  if (!c || !x_dev) return fail(DEX_EINVAL, "null argument");
+  DEX_HIP(hipSetDevice(c->p.device));
+  for (int j = 0; j < ncols; ++j) {
+    Dbl2 *xj = static_cast<Dbl2 *>(x_dev) + size_t(j) * c->d;
+    DEX_HIP(dex::launch_fill_gaussian(xj, c->d, seed, uint32_t(j), stream_id, c->stream));
+    if (singlet) DEX_HIP(dex::launch_singlet_project(xj, c->m, c->stream));
+  }
+  DEX_HIP(hipStreamSynchronize(c->stream));
+  return DEX_OK;
+}
+
+extern "C" int dex_sync(dex_ctx *c) {
+  if (!c) return fail(DEX_EINVAL, "null argument");
+  DEX_HIP(hipStreamSynchronize(c->stream));
+  return DEX_OK;
+}
+
+extern "C" int dex_host_kematrix(int32_t nn, double delta, double *t) {
+  if (nn < 1 || !(delta > 0) || !t) return fail(DEX_EINVAL, "invalid argument");
+  dex::kematrix(nn, delta, t);
+  return DEX_OK;
+}
+
+extern "C" int dex_host_getinverse(int32_t nout, double delta, int32_t nsmall, int32_t tfactor,
+                                   double *tinv) {
+  if (nout < 0 || !(delta > 0) || !tinv) return fail(DEX_EINVAL, "invalid argument");
+  std::vector<double> t;
+  if (dex::getinverse(nout, delta, nsmall, tfactor, t) != 0) return fail(DEX_EINVAL, "nbig < nout");
+  std::memcpy(tinv, t.data(), t.size() * sizeof(double));
+  return DEX_OK;
+}
+
+extern "C" int dex_host_solve_gevp(int32_t k, const double *kp, const double *mp, int32_t ld,
+                                   int32_t nev, double *coords, double *mu) {
+  if (k < 1 || nev < 1 || nev > k || ld < k || !kp || !mp || !coords || !mu)
+    return fail(DEX_EINVAL, "invalid argument");
+  std::vector<cd> cv, mv;
+  const int st = dex::harmonic_pencil(k, reinterpret_cast<const cd *>(kp), reinterpret_cast<const cd *>(mp), ld,
+                                      nev, cv, mv);
+  if (st != 0) return fail(DEX_EGEVP, "SOLVE OF THE PROJECTED EIGENVALUE PROBLEM FAILED");
+  std::memcpy(coords, cv.data(), cv.size() * sizeof(cd));
+  std::memcpy(mu, mv.data(), mv.size() * sizeof(cd));
+  return DEX_OK;
+}

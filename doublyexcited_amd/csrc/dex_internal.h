@@ -1,0 +1,92 @@
+// Internal declarations shared by the host sources and the HIP kernels.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/dex.h"
+
+namespace dex {
+
+// ---- host set-up (dex_setup.cpp) -----------------------------------------
+double kinetic_tau(int m);
+void kematrix(int nn, double delta, double *t);
+int getinverse(int nout, double delta, int nsmall, int tfactor, std::vector<double> &tinv);
+
+// ---- small dense LA (dex_smallla.cpp) -------------------------------------
+int zpotrf_upper(int n, std::complex<double> *a, int lda);
+void ztrtri_upper(int n, std::complex<double> *r, int ldr);
+void zheev_jacobi(int n, const std::complex<double> *a, int lda, std::vector<double> &w,
+                  std::vector<std::complex<double>> &v);
+int zgeev_right(int n, const std::complex<double> *a, int lda, std::vector<std::complex<double>> &lam,
+                std::vector<std::complex<double>> &vec);
+int harmonic_pencil(int k, const std::complex<double> *kp, const std::complex<double> *mp, int ld,
+                    int nev, std::vector<std::complex<double>> &coords,
+                    std::vector<std::complex<double>> &mu);
+
+// ---- device kernels (dex_kernels.hip) ---------------------------------------
+struct Dbl2 {
+  double x, y;
+};
+
+// One column of a tall block as seen by the block kernels: the value at row r
+// is a[r] - sig * b[r] (b may be null).  Used to form A Q - sigma Q on the fly.
+struct ColDesc {
+  const Dbl2 *a;
+  const Dbl2 *b;
+  double sig_re, sig_im;
+};
+
+struct MatvecCoef {
+  double s2_re, s2_im;   // e^{-2 i theta} / delta^2 multiplying the kinetic sum
+  double c1_re, c1_im;   // e^{-i theta} multiplying the real potential
+  double inv_d3;         // 1 / delta^3 (electron-electron term)
+};
+
+// Returns false if no fast (LDS two-pass) kernel exists for (nn, single).
+bool fast_matvec_supported(int nn, bool single);
+bool generic_matvec_supported(int nn);
+
+// Fast path: U = K1 X (electron 1), then Y = s2 (U + K2 X) + c1 Vr X.
+hipError_t launch_pass1(int nn, bool single, const void *x, void *u, hipStream_t s);
+hipError_t launch_pass2(int nn, bool single, const void *x, const void *u, void *y,
+                        const double *vn, const double *tinv, const MatvecCoef &coef,
+                        hipStream_t s);
+// Generic FP64 path for any odd nn <= 27: six axis passes plus the potential.
+hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *vn,
+                                 const double *tinv, const MatvecCoef &coef, hipStream_t s);
+
+// G(kl x kr, column-major) = L^H R summed over d rows.  `partial` must hold
+// gram_partial_elems(kl, kr) Dbl2; `descs` is a device array of kl + kr
+// descriptors (L first).
+int gram_blocks();
+size_t gram_partial_elems(int kl, int kr);
+hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int64_t d, Dbl2 *partial, Dbl2 *out,
+                       hipStream_t s);
+
+// out_j = beta_j out_j + sum_i in_i C(i, j) row by row; in-place aliasing of
+// an output with an input is allowed.  `descs`: nin input descriptors then
+// nout output pointers (as ColDesc.a); C (nin x nout) and beta (nout, may be
+// null) are device arrays.
+hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c,
+                          const Dbl2 *beta, int64_t d, hipStream_t s);
+
+// Per column j: sum |a_j|^2, sum conj(a_j) b_j, sum |b_j|^2 (b from descs[k + j]).
+size_t coldots_partial_elems(int k);
+hipError_t launch_coldots(const ColDesc *descs, int k, int64_t d, double *partial, double *out,
+                          hipStream_t s);
+
+// Philox4x32-10 complex Gaussian fill: element i of column c gets
+// (g0 + i g1)/sqrt(2) with g from Box-Muller on the counter (i, c, stream, 0).
+hipError_t launch_fill_gaussian(Dbl2 *x, int64_t n, uint64_t seed, uint32_t col, uint32_t stream_id,
+                                hipStream_t s);
+// Singlet projection X <- (X + X^T)/2 in the M x M view (in place).
+hipError_t launch_singlet_project(Dbl2 *x, int64_t m, hipStream_t s);
+hipError_t launch_to_single(const Dbl2 *x, void *xs, int64_t n, hipStream_t s);
+hipError_t launch_to_double(const void *xs, Dbl2 *x, int64_t n, hipStream_t s);
+
+}  // namespace dex

@@ -1,0 +1,764 @@
+// HIP kernels for gfx950 (MI355X): the two-electron sinc-DVR matvec and the
+// tall-skinny block kernels of the GPLMR eigensolver.
+//
+// Matvec.  A two-electron vector is the M x M matrix X[p + M q] (p: electron 1,
+// q: electron 2, each p = i + NN j + NN^2 k).  With T = tau / delta^2 the real
+// Toeplitz sinc-DVR matrix (tau(0) = pi^2/6, tau(m) = (-1)^m / m^2) the
+// reference's HAMMULT (hammod.F90:19-67) is
+//     Y = e^{-2 i theta} (T3D x I + I x T3D) X + e^{-i theta} Vr o X,
+// Vr(p, q) = -Z/delta^3 (Tinv(r_p) + Tinv(r_q)) + Tinv(r_q - r_p)/delta^3.
+// tau is a compile-time table, so the kinetic contraction is a stream of
+// FMAs with scalar constants.  Two passes, each over an NN^3 cube held in LDS:
+//   pass 1 (electron 1): U[:, q] = tau3D X[:, q]  (contiguous cubes),
+//   pass 2 (electron 2): Y[p, :] = s2 (U[p, :] + tau3D X[p, :]) + c1 Vr X[p, :]
+// (cubes at stride M; neighbouring p share cache lines, so the workgroup ->
+// p map keeps groups of 8 consecutive p on one XCD).  The kinetic operator is
+// real, so the real and imaginary parts are contracted in turn through two
+// LDS buffers of NN^3 reals each (148 KB at NN = 21 in FP64).
+#include <hip/hip_runtime.h>
+
+#include "dex_internal.h"
+
+namespace dex {
+
+namespace {
+
+constexpr double kPi2over6 = 1.6449340668482264;  // pi^2 / 6
+
+template <int M>
+__device__ __forceinline__ constexpr double tau_c() {
+  if constexpr (M == 0) {
+    return kPi2over6;
+  } else {
+    return ((M & 1) ? -1.0 : 1.0) / double(M * M);
+  }
+}
+
+__device__ __forceinline__ constexpr double tau_rt(int m) {
+  m = m < 0 ? -m : m;
+  return m == 0 ? kPi2over6 : ((m & 1) ? -1.0 : 1.0) / double(m * m);
+}
+
+template <typename R>
+struct Cplx;
+template <>
+struct Cplx<double> {
+  using type = double2;
+};
+template <>
+struct Cplx<float> {
+  using type = float2;
+};
+
+constexpr int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+template <int NN>
+constexpr int cube_threads() {
+  return round_up(NN * NN, 64) > 1024 ? 1024 : round_up(NN * NN, 64);
+}
+
+// b = tau3D a on one NN^3 cube in LDS (a, b: NN^3 reals, i fastest).
+// One thread per line: it reads the NN inputs of its line once and forms all
+// NN outputs with NN^2 FMAs against compile-time coefficients.
+template <int NN, int NT, typename R>
+__device__ __forceinline__ void cube_apply(const R *__restrict__ a, R *__restrict__ b, int tid) {
+  constexpr int N2 = NN * NN;
+  // axis i (stride 1): line c = (j, k), base c * NN
+  for (int c = tid; c < N2; c += NT) {
+    R acc[NN];
+#pragma unroll
+    for (int i = 0; i < NN; ++i) acc[i] = R(0);
+#pragma unroll
+    for (int l = 0; l < NN; ++l) {
+      const R xv = a[c * NN + l];
+#pragma unroll
+      for (int i = 0; i < NN; ++i) acc[i] += R(tau_rt(l - i)) * xv;
+    }
+#pragma unroll
+    for (int i = 0; i < NN; ++i) b[c * NN + i] = acc[i];
+  }
+  __syncthreads();
+  // axis j (stride NN): line c = (i, k)
+  for (int c = tid; c < N2; c += NT) {
+    const int i = c % NN, k = c / NN;
+    const int base = i + N2 * k;
+    R acc[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) acc[j] = R(0);
+#pragma unroll
+    for (int l = 0; l < NN; ++l) {
+      const R xv = a[base + NN * l];
+#pragma unroll
+      for (int j = 0; j < NN; ++j) acc[j] += R(tau_rt(l - j)) * xv;
+    }
+#pragma unroll
+    for (int j = 0; j < NN; ++j) b[base + NN * j] += acc[j];
+  }
+  __syncthreads();
+  // axis k (stride NN^2): line c = (i, j)
+  for (int c = tid; c < N2; c += NT) {
+    R acc[NN];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) acc[k] = R(0);
+#pragma unroll
+    for (int l = 0; l < NN; ++l) {
+      const R xv = a[c + N2 * l];
+#pragma unroll
+      for (int k = 0; k < NN; ++k) acc[k] += R(tau_rt(l - k)) * xv;
+    }
+#pragma unroll
+    for (int k = 0; k < NN; ++k) b[c + N2 * k] += acc[k];
+  }
+  __syncthreads();
+}
+
+// Pass 1: one workgroup per electron-2 index q; its electron-1 cube is
+// contiguous.  U = tau3D X on that cube (no phase, no potential).
+template <int NN, typename R>
+__global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(
+    const typename Cplx<R>::type *__restrict__ x, typename Cplx<R>::type *__restrict__ u) {
+  using C = typename Cplx<R>::type;
+  constexpr int NT = cube_threads<NN>();
+  constexpr int N3 = NN * NN * NN;
+  constexpr int PER = (N3 + NT - 1) / NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  R *a = reinterpret_cast<R *>(smem);
+  R *b = a + N3;
+  const int tid = threadIdx.x;
+  const int64_t q = blockIdx.x;
+  const C *xq = x + q * N3;
+  C *uq = u + q * N3;
+  R keep[PER];
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int e = tid + t * NT;
+    if (e < N3) {
+      const C v = xq[e];
+      a[e] = v.x;
+      keep[t] = v.y;
+    }
+  }
+  __syncthreads();
+  cube_apply<NN, NT, R>(a, b, tid);
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int e = tid + t * NT;
+    if (e < N3) {
+      const R re = b[e];
+      a[e] = keep[t];
+      keep[t] = re;
+    }
+  }
+  __syncthreads();
+  cube_apply<NN, NT, R>(a, b, tid);
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int e = tid + t * NT;
+    if (e < N3) {
+      C o;
+      o.x = keep[t];
+      o.y = b[e];
+      uq[e] = o;
+    }
+  }
+}
+
+// Workgroup -> electron-1 index p for pass 2: consecutive workgroups are
+// dealt round-robin to the 8 XCDs, so the s-th workgroup on XCD x takes
+// p = 8 (x + 8 (s / 8)) + s % 8, which puts 8 consecutive p (one 128-byte
+// line of complex doubles per q) on the same XCD at nearly the same time.
+__device__ __forceinline__ int64_t xcd_group_map(int64_t b, int64_t m) {
+  const int64_t full = (m / 64) * 64;
+  if (b >= full) return b;
+  const int64_t x = b % 8, s = b / 8;
+  return 8 * (x + 8 * (s / 8)) + (s % 8);
+}
+
+template <int NN, typename R>
+__global__ __launch_bounds__(cube_threads<NN>()) void k_pass2(
+    const typename Cplx<R>::type *__restrict__ x, const typename Cplx<R>::type *__restrict__ u,
+    typename Cplx<R>::type *__restrict__ y, const double *__restrict__ vn,
+    const double *__restrict__ tinv, MatvecCoef cf) {
+  using C = typename Cplx<R>::type;
+  constexpr int NT = cube_threads<NN>();
+  constexpr int N3 = NN * NN * NN;
+  constexpr int PER = (N3 + NT - 1) / NT;
+  constexpr int L = 2 * NN + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  R *a = reinterpret_cast<R *>(smem);
+  R *b = a + N3;
+  const int tid = threadIdx.x;
+  const int64_t p = xcd_group_map(blockIdx.x, N3);
+  const int64_t M = N3;
+  R keep[PER];
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int e = tid + t * NT;
+    if (e < N3) {
+      const C v = x[p + M * e];
+      a[e] = v.x;
+      keep[t] = v.y;
+    }
+  }
+  __syncthreads();
+  cube_apply<NN, NT, R>(a, b, tid);
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int e = tid + t * NT;
+    if (e < N3) {
+      const R re = b[e];
+      a[e] = keep[t];
+      keep[t] = re;
+    }
+  }
+  __syncthreads();
+  cube_apply<NN, NT, R>(a, b, tid);
+  // epilogue: Y = s2 (U + K) + c1 Vr X
+  const int ip = int(p % NN), jp = int((p / NN) % NN), kp = int(p / (NN * NN));
+  const double vnp = vn[p];
+  const R s2r = R(cf.s2_re), s2i = R(cf.s2_im), c1r = R(cf.c1_re), c1i = R(cf.c1_im);
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int e = tid + t * NT;
+    if (e < N3) {
+      const int iq = e % NN, jq = (e / NN) % NN, kq = e / (NN * NN);
+      const int ti = (iq - ip + NN) + L * ((jq - jp + NN) + L * (kq - kp + NN));
+      const R vr = R(vnp + vn[e] + tinv[ti] * cf.inv_d3);
+      const C uv = u[p + M * e];
+      const C xv = x[p + M * e];
+      const R sr = uv.x + keep[t], si = uv.y + b[e];
+      const R vxr = vr * xv.x, vxi = vr * xv.y;
+      C o;
+      o.x = s2r * sr - s2i * si + c1r * vxr - c1i * vxi;
+      o.y = s2r * si + s2i * sr + c1r * vxi + c1i * vxr;
+      y[p + M * e] = o;
+    }
+  }
+}
+
+// ---- generic FP64 path ------------------------------------------------------
+// One thread per line of the given axis (stride s): y (+)= tau * x along it.
+template <int NN>
+__global__ __launch_bounds__(256) void k_axis(const double2 *__restrict__ x, double2 *__restrict__ y,
+                                              int64_t stride, int64_t nlines, int accumulate) {
+  const int64_t line = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  const int64_t off = (line % stride) + (line / stride) * stride * NN;
+  double2 xv[NN];
+#pragma unroll
+  for (int l = 0; l < NN; ++l) xv[l] = x[off + stride * l];
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    double ar = 0.0, ai = 0.0;
+#pragma unroll
+    for (int l = 0; l < NN; ++l) {
+      const double t = tau_rt(l - i);
+      ar += t * xv[l].x;
+      ai += t * xv[l].y;
+    }
+    double2 o = accumulate ? y[off + stride * i] : make_double2(0.0, 0.0);
+    o.x += ar;
+    o.y += ai;
+    y[off + stride * i] = o;
+  }
+}
+
+__global__ void k_potential(const double2 *__restrict__ x, double2 *__restrict__ y,
+                            const double *__restrict__ vn, const double *__restrict__ tinv, int nn,
+                            int64_t m, MatvecCoef cf) {
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= m * m) return;
+  const int64_t p = idx % m, q = idx / m;
+  const int L = 2 * nn + 1;
+  const int ip = int(p % nn), jp = int((p / nn) % nn), kp = int(p / (int64_t(nn) * nn));
+  const int iq = int(q % nn), jq = int((q / nn) % nn), kq = int(q / (int64_t(nn) * nn));
+  const int64_t ti = (iq - ip + nn) + int64_t(L) * ((jq - jp + nn) + int64_t(L) * (kq - kp + nn));
+  const double vr = vn[p] + vn[q] + tinv[ti] * cf.inv_d3;
+  const double2 xv = x[idx];
+  const double2 kv = y[idx];
+  double2 o;
+  o.x = cf.s2_re * kv.x - cf.s2_im * kv.y + vr * (cf.c1_re * xv.x - cf.c1_im * xv.y);
+  o.y = cf.s2_re * kv.y + cf.s2_im * kv.x + vr * (cf.c1_re * xv.y + cf.c1_im * xv.x);
+  y[idx] = o;
+}
+
+template <int NN>
+hipError_t matvec_generic_nn(const double2 *x, double2 *y, const double *vn, const double *tinv,
+                             const MatvecCoef &cf, hipStream_t s) {
+  const int64_t m = int64_t(NN) * NN * NN;
+  const int64_t d = m * m;
+  const int64_t nlines = d / NN;
+  const int bs = 256;
+  const int64_t nb = (nlines + bs - 1) / bs;
+  int64_t stride = 1;
+  for (int axis = 0; axis < 6; ++axis) {
+    hipLaunchKernelGGL(k_axis<NN>, dim3(unsigned(nb)), dim3(bs), 0, s, x, y, stride, nlines,
+                       axis == 0 ? 0 : 1);
+    stride *= NN;
+  }
+  const int64_t nb2 = (d + bs - 1) / bs;
+  hipLaunchKernelGGL(k_potential, dim3(unsigned(nb2)), dim3(bs), 0, s, x, y, vn, tinv, NN, m, cf);
+  return hipGetLastError();
+}
+
+template <int NN, typename R>
+hipError_t pass1_nn(const void *x, void *u, hipStream_t s) {
+  using C = typename Cplx<R>::type;
+  constexpr int N3 = NN * NN * NN;
+  const size_t lds = size_t(2) * N3 * sizeof(R);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass1<NN, R>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_pass1<NN, R>), dim3(N3), dim3(cube_threads<NN>()), lds, s,
+                     static_cast<const C *>(x), static_cast<C *>(u));
+  return hipGetLastError();
+}
+
+template <int NN, typename R>
+hipError_t pass2_nn(const void *x, const void *u, void *y, const double *vn, const double *tinv,
+                    const MatvecCoef &cf, hipStream_t s) {
+  using C = typename Cplx<R>::type;
+  constexpr int N3 = NN * NN * NN;
+  const size_t lds = size_t(2) * N3 * sizeof(R);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass2<NN, R>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_pass2<NN, R>), dim3(N3), dim3(cube_threads<NN>()), lds, s,
+                     static_cast<const C *>(x), static_cast<const C *>(u), static_cast<C *>(y), vn,
+                     tinv, cf);
+  return hipGetLastError();
+}
+
+// LDS budget per workgroup on gfx950.
+constexpr size_t kMaxLds = 160 * 1024;
+
+constexpr bool fast_fits(int nn, size_t rsize) {
+  return size_t(2) * nn * nn * nn * rsize <= kMaxLds;
+}
+
+}  // namespace
+
+#define DEX_NN_LIST(X) X(3) X(5) X(7) X(9) X(11) X(13) X(15) X(17) X(19) X(21) X(23) X(25)
+
+bool fast_matvec_supported(int nn, bool single) {
+  switch (nn) {
+#define CASE(N) \
+  case N:       \
+    return fast_fits(N, single ? sizeof(float) : sizeof(double));
+    DEX_NN_LIST(CASE)
+#undef CASE
+    default:
+      return false;
+  }
+}
+
+bool generic_matvec_supported(int nn) {
+  switch (nn) {
+#define CASE(N) \
+  case N:       \
+    return true;
+    DEX_NN_LIST(CASE)
+#undef CASE
+    default:
+      return false;
+  }
+}
+
+hipError_t launch_pass1(int nn, bool single, const void *x, void *u, hipStream_t s) {
+  switch (nn) {
+#define CASE(N)                                                                             \
+  case N:                                                                                   \
+    if constexpr (fast_fits(N, sizeof(double))) {                                           \
+      if (!single) return pass1_nn<N, double>(x, u, s);                                     \
+    }                                                                                       \
+    if constexpr (fast_fits(N, sizeof(float))) {                                            \
+      if (single) return pass1_nn<N, float>(x, u, s);                                       \
+    }                                                                                       \
+    return hipErrorInvalidValue;
+    DEX_NN_LIST(CASE)
+#undef CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pass2(int nn, bool single, const void *x, const void *u, void *y,
+                        const double *vn, const double *tinv, const MatvecCoef &coef,
+                        hipStream_t s) {
+  switch (nn) {
+#define CASE(N)                                                                             \
+  case N:                                                                                   \
+    if constexpr (fast_fits(N, sizeof(double))) {                                           \
+      if (!single) return pass2_nn<N, double>(x, u, y, vn, tinv, coef, s);                  \
+    }                                                                                       \
+    if constexpr (fast_fits(N, sizeof(float))) {                                            \
+      if (single) return pass2_nn<N, float>(x, u, y, vn, tinv, coef, s);                    \
+    }                                                                                       \
+    return hipErrorInvalidValue;
+    DEX_NN_LIST(CASE)
+#undef CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *vn,
+                                 const double *tinv, const MatvecCoef &coef, hipStream_t s) {
+  const double2 *xx = reinterpret_cast<const double2 *>(x);
+  double2 *yy = reinterpret_cast<double2 *>(y);
+  switch (nn) {
+#define CASE(N) \
+  case N:       \
+    return matvec_generic_nn<N>(xx, yy, vn, tinv, coef, s);
+    DEX_NN_LIST(CASE)
+#undef CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// ---- block kernels ----------------------------------------------------------
+namespace {
+
+constexpr int kGramThreads = 256;
+constexpr int kGramRows = 32;        // rows per LDS tile
+constexpr int kGramMaxCols = 192;    // kl + kr
+constexpr int kGramTilesPerThread = 2;
+constexpr int kGramBlocks = 1024;
+
+__device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
+  const double2 va = reinterpret_cast<const double2 *>(c.a)[r];
+  if (c.b == nullptr) return va;
+  const double2 vb = reinterpret_cast<const double2 *>(c.b)[r];
+  double2 o;
+  o.x = va.x - (c.sig_re * vb.x - c.sig_im * vb.y);
+  o.y = va.y - (c.sig_re * vb.y + c.sig_im * vb.x);
+  return o;
+}
+
+// Partial Gram over a contiguous row range per workgroup.  Output tiles of
+// 4 x 4 complex entries are dealt to threads; LDS holds kGramRows rows of all
+// kl + kr columns.
+__global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict__ descs, int kl,
+                                                       int kr, int64_t d, int64_t rows_per_block,
+                                                       double2 *__restrict__ partial) {
+  __shared__ double2 tile[kGramRows][kGramMaxCols + 1];
+  const int tid = threadIdx.x;
+  const int ncol = kl + kr;
+  const int tl = (kl + 3) / 4, trr = (kr + 3) / 4;
+  const int ntiles = tl * trr;
+  double2 acc[kGramTilesPerThread][4][4];
+#pragma unroll
+  for (int t = 0; t < kGramTilesPerThread; ++t)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[t][a][b] = make_double2(0.0, 0.0);
+  const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
+  for (int64_t rb = r0; rb < r1; rb += kGramRows) {
+    const int nr = int(r1 - rb < kGramRows ? r1 - rb : kGramRows);
+    for (int e = tid; e < kGramRows * ncol; e += kGramThreads) {
+      const int r = e % kGramRows, c = e / kGramRows;
+      tile[r][c] = r < nr ? col_value(descs[c], rb + r) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kGramTilesPerThread; ++t) {
+      const int tix = tid + t * kGramThreads;
+      if (tix < ntiles) {
+        const int i0 = (tix % tl) * 4, j0 = (tix / tl) * 4;
+        for (int r = 0; r < kGramRows; ++r) {
+          double2 lv[4], rv[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) lv[a] = tile[r][i0 + a < kl ? i0 + a : 0];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) rv[b] = tile[r][kl + (j0 + b < kr ? j0 + b : 0)];
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+              // conj(l) * r
+              acc[t][a][b].x += lv[a].x * rv[b].x + lv[a].y * rv[b].y;
+              acc[t][a][b].y += lv[a].x * rv[b].y - lv[a].y * rv[b].x;
+            }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double2 *out = partial + size_t(blockIdx.x) * kl * kr;
+#pragma unroll
+  for (int t = 0; t < kGramTilesPerThread; ++t) {
+    const int tix = tid + t * kGramThreads;
+    if (tix < ntiles) {
+      const int i0 = (tix % tl) * 4, j0 = (tix / tl) * 4;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (i0 + a < kl && j0 + b < kr) out[(i0 + a) + size_t(kl) * (j0 + b)] = acc[t][a][b];
+    }
+  }
+}
+
+__global__ void k_reduce_partials(const double2 *__restrict__ partial, int nparts, int nelem,
+                                  double2 *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  double sx = 0.0, sy = 0.0;
+  for (int b = 0; b < nparts; ++b) {
+    const double2 v = partial[size_t(b) * nelem + e];
+    sx += v.x;
+    sy += v.y;
+  }
+  out[e] = make_double2(sx, sy);
+}
+
+constexpr int kCombRows = 64;
+constexpr int kCombThreads = 256;
+constexpr int kCombMaxIn = 96;
+constexpr int kCombMaxOut = 48;
+
+__global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restrict__ descs, int nin,
+                                                          int nout, const double2 *__restrict__ cm,
+                                                          const double2 *__restrict__ beta, int64_t d) {
+  __shared__ double2 tin[kCombRows][kCombMaxIn + 1];
+  const int tid = threadIdx.x;
+  const int64_t ntile = (d + kCombRows - 1) / kCombRows;
+  for (int64_t tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
+    const int64_t rb = tb * kCombRows;
+    const int nr = int(d - rb < kCombRows ? d - rb : kCombRows);
+    for (int e = tid; e < kCombRows * nin; e += kCombThreads) {
+      const int r = e % kCombRows, c = e / kCombRows;
+      if (r < nr) tin[r][c] = col_value(descs[c], rb + r);
+    }
+    __syncthreads();
+    for (int e = tid; e < kCombRows * nout; e += kCombThreads) {
+      const int r = e % kCombRows, j = e / kCombRows;
+      if (r < nr) {
+        double sx = 0.0, sy = 0.0;
+        for (int i = 0; i < nin; ++i) {
+          const double2 a = tin[r][i];
+          const double2 c = cm[i + size_t(nin) * j];
+          sx += a.x * c.x - a.y * c.y;
+          sy += a.x * c.y + a.y * c.x;
+        }
+        double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + j].a)) + rb + r;
+        if (beta != nullptr) {
+          const double2 bj = beta[j];
+          const double2 ov = *o;
+          sx += bj.x * ov.x - bj.y * ov.y;
+          sy += bj.x * ov.y + bj.y * ov.x;
+        }
+        *o = make_double2(sx, sy);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+constexpr int kDotThreads = 256;
+constexpr int kDotBlocks = 512;
+constexpr int kDotMaxCols = 64;
+
+// Per column: |a|^2, conj(a) b (re, im), |b|^2 -> 4 doubles per column per block.
+__global__ __launch_bounds__(kDotThreads) void k_coldots(const ColDesc *__restrict__ descs, int k,
+                                                         int64_t d, double *__restrict__ partial) {
+  __shared__ double red[kDotThreads / 64][4];
+  const int tid = threadIdx.x;
+  for (int j = 0; j < k; ++j) {
+    const ColDesc ca = descs[j];
+    const ColDesc cb = descs[k + j];
+    double aa = 0.0, abr = 0.0, abi = 0.0, bb = 0.0;
+    for (int64_t r = int64_t(blockIdx.x) * kDotThreads + tid; r < d; r += int64_t(gridDim.x) * kDotThreads) {
+      const double2 a = col_value(ca, r);
+      const double2 b = cb.a != nullptr ? col_value(cb, r) : make_double2(0.0, 0.0);
+      aa += a.x * a.x + a.y * a.y;
+      abr += a.x * b.x + a.y * b.y;
+      abi += a.x * b.y - a.y * b.x;
+      bb += b.x * b.x + b.y * b.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      aa += __shfl_down(aa, off);
+      abr += __shfl_down(abr, off);
+      abi += __shfl_down(abi, off);
+      bb += __shfl_down(bb, off);
+    }
+    if ((tid & 63) == 0) {
+      red[tid >> 6][0] = aa;
+      red[tid >> 6][1] = abr;
+      red[tid >> 6][2] = abi;
+      red[tid >> 6][3] = bb;
+    }
+    __syncthreads();
+    if (tid < 4) {
+      double s = 0.0;
+      for (int w = 0; w < kDotThreads / 64; ++w) s += red[w][tid];
+      partial[(size_t(blockIdx.x) * k + j) * 4 + tid] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_reduce_real(const double *__restrict__ partial, int nparts, int nelem,
+                              double *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  double s = 0.0;
+  for (int b = 0; b < nparts; ++b) s += partial[size_t(b) * nelem + e];
+  out[e] = s;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11) and Box-Muller.
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(M0) * c[0];
+    const uint64_t p1 = uint64_t(M1) * c[2];
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0;
+    const uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+__global__ void k_fill_gaussian(double2 *__restrict__ x, int64_t n, uint64_t seed, uint32_t col,
+                                uint32_t sid) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c[4] = {uint32_t(i), uint32_t(uint64_t(i) >> 32), col, sid};
+  philox10(c, uint32_t(seed), uint32_t(seed >> 32));
+  // 53-bit uniforms in (0, 1]
+  const double u1 = (double((uint64_t(c[0]) << 21) ^ (c[1] >> 11)) + 1.0) * (1.0 / 9007199254740992.0);
+  const double u2 = double((uint64_t(c[2]) << 21) ^ (c[3] >> 11)) * (1.0 / 9007199254740992.0);
+  const double rad = sqrt(-2.0 * log(u1));
+  const double ang = 6.283185307179586 * u2;
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  const double s = 0.7071067811865476;
+  x[i] = make_double2(rad * cs * s, rad * sn * s);
+}
+
+// X <- (X + X^T)/2 on the M x M matrix, tiles of 32 x 32 above the diagonal.
+__global__ void k_singlet(double2 *__restrict__ x, int64_t m) {
+  __shared__ double2 ta[32][33], tb[32][33];
+  const int64_t bi = blockIdx.x, bj = blockIdx.y;
+  if (bi > bj) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t i = bi * 32 + tx, j = bj * 32 + r;  // element (i, j): x[i + m j]
+    ta[r][tx] = (i < m && j < m) ? x[i + m * j] : make_double2(0, 0);
+    const int64_t i2 = bj * 32 + tx, j2 = bi * 32 + r;
+    tb[r][tx] = (i2 < m && j2 < m) ? x[i2 + m * j2] : make_double2(0, 0);
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t i = bi * 32 + tx, j = bj * 32 + r;
+    const double2 a = ta[r][tx], b = tb[tx][r];  // b = x[j + m i]
+    const double2 s = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+    if (i < m && j < m) x[i + m * j] = s;
+    const int64_t i2 = bj * 32 + tx, j2 = bi * 32 + r;
+    const double2 a2 = tb[r][tx], b2 = ta[tx][r];
+    const double2 s2 = make_double2(0.5 * (a2.x + b2.x), 0.5 * (a2.y + b2.y));
+    if (i2 < m && j2 < m) x[i2 + m * j2] = s2;
+  }
+}
+
+__global__ void k_to_single(const double2 *__restrict__ x, float2 *__restrict__ y, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = make_float2(float(x[i].x), float(x[i].y));
+}
+
+__global__ void k_to_double(const float2 *__restrict__ x, double2 *__restrict__ y, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = make_double2(double(x[i].x), double(x[i].y));
+}
+
+}  // namespace
+
+int gram_blocks() { return kGramBlocks; }
+
+size_t gram_partial_elems(int kl, int kr) { return size_t(kGramBlocks) * kl * kr; }
+
+hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int64_t d, Dbl2 *partial, Dbl2 *out,
+                       hipStream_t s) {
+  if (kl < 1 || kr < 1 || kl + kr > kGramMaxCols) return hipErrorInvalidValue;
+  if (((kl + 3) / 4) * ((kr + 3) / 4) > kGramThreads * kGramTilesPerThread) return hipErrorInvalidValue;
+  int64_t rpb = (d + kGramBlocks - 1) / kGramBlocks;
+  rpb = (rpb + kGramRows - 1) / kGramRows * kGramRows;
+  hipLaunchKernelGGL(k_gram, dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr, d, rpb,
+                     reinterpret_cast<double2 *>(partial));
+  const int nelem = kl * kr;
+  hipLaunchKernelGGL(k_reduce_partials, dim3((nelem + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<const double2 *>(partial), kGramBlocks, nelem,
+                     reinterpret_cast<double2 *>(out));
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
+                          int64_t d, hipStream_t s) {
+  if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombMaxOut) return hipErrorInvalidValue;
+  const int64_t ntile = (d + kCombRows - 1) / kCombRows;
+  const unsigned grid = unsigned(ntile < 4096 ? ntile : 4096);
+  hipLaunchKernelGGL(k_combine, dim3(grid), dim3(kCombThreads), 0, s, descs, nin, nout,
+                     reinterpret_cast<const double2 *>(c), reinterpret_cast<const double2 *>(beta), d);
+  return hipGetLastError();
+}
+
+size_t coldots_partial_elems(int k) { return size_t(kDotBlocks) * k * 4; }
+
+hipError_t launch_coldots(const ColDesc *descs, int k, int64_t d, double *partial, double *out,
+                          hipStream_t s) {
+  if (k < 1 || k > kDotMaxCols) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_coldots, dim3(kDotBlocks), dim3(kDotThreads), 0, s, descs, k, d, partial);
+  const int nelem = k * 4;
+  hipLaunchKernelGGL(k_reduce_real, dim3((nelem + 255) / 256), dim3(256), 0, s, partial, kDotBlocks,
+                     nelem, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_gaussian(Dbl2 *x, int64_t n, uint64_t seed, uint32_t col, uint32_t stream_id,
+                                hipStream_t s) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_fill_gaussian, dim3(unsigned((n + bs - 1) / bs)), dim3(bs), 0, s,
+                     reinterpret_cast<double2 *>(x), n, seed, col, stream_id);
+  return hipGetLastError();
+}
+
+hipError_t launch_singlet_project(Dbl2 *x, int64_t m, hipStream_t s) {
+  const unsigned nb = unsigned((m + 31) / 32);
+  hipLaunchKernelGGL(k_singlet, dim3(nb, nb), dim3(32, 8), 0, s, reinterpret_cast<double2 *>(x), m);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_single(const Dbl2 *x, void *xs, int64_t n, hipStream_t s) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_to_single, dim3(unsigned((n + bs - 1) / bs)), dim3(bs), 0, s,
+                     reinterpret_cast<const double2 *>(x), static_cast<float2 *>(xs), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_double(const void *xs, Dbl2 *x, int64_t n, hipStream_t s) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_to_double, dim3(unsigned((n + bs - 1) / bs)), dim3(bs), 0, s,
+                     static_cast<const float2 *>(xs), reinterpret_cast<double2 *>(x), n);
+  return hipGetLastError();
+}
+
+}  // namespace dex

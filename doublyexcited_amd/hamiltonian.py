@@ -1,0 +1,223 @@
+"""Host-side mirror of the reference's operator / eigensolver interface.
+
+``HeliumGrid`` owns one native context (``dex_ctx``): the reference's
+``PARAMETERS`` + ``HAMILTONIAN`` built by ``INIT_STRUCTS(NELEC=2)``
+(input.F90:365-417).  Its methods keep the reference's argument meaning:
+
+* ``hammult(x)``                 -> HAMMULT(HAM,PARAMS,MPIDATA,X,Y,MM), hammod.F90:19
+* ``mymatmul(inp, out, howmany, checksize)`` -> MYMATMUL, doublyexcited.F90:24-44
+* ``mydotsub(bras, kets, size, nn, mm)``     -> MYDOTSUB, doublyexcited.F90:47-59
+* ``gplmr_eig(x, sigma, nev, ns, maxit, tol, inner_iter, outmodulus)``
+                                 -> GPLMR_EIG, gplmr.F90:9
+
+Host arrays are numpy complex128, vector-major (vector j is ``x[j*D:(j+1)*D]``
+or column j of a (D, k) Fortran-order array).  Device variants take torch
+tensors on ``cuda`` (torch provides the memory; all arithmetic is in the
+native library).  Size mismatches raise ``DexError`` with status ``DEX_ESIZE``
+where the reference prints "SIZE ERRORRRR 66" and STOPs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from ._lib import DexError, check
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+@dataclasses.dataclass
+class SolveResult:
+    e: np.ndarray
+    residuals: np.ndarray | None
+    iterations: int
+    converged: bool
+    matvecs: int
+    max_residual: float
+    t_total: float
+    t_matvec: float
+    t_gram: float
+    t_update: float
+    t_smalleig: float
+    fallbacks: int
+    fp64_restarts_used: int
+    status: int
+
+
+class HeliumGrid:
+    """One complex-scaled two-electron sinc-DVR Hamiltonian on one GPU."""
+
+    def __init__(self, n: int, delta: float, theta: float, charge: int = 2, nsmall: int = 22,
+                 tfactor: int = 4, precision: str = "fp64", device: int = 0):
+        lib = _lib.load()
+        p = _lib.DexParams()
+        lib.dex_params_default(C.byref(p))
+        p.n, p.delta, p.theta, p.charge = n, float(delta), float(theta), charge
+        p.nsmall, p.tfactor, p.device = nsmall, tfactor, device
+        p.precision = {"fp64": _lib.DEX_FP64, "fp32": _lib.DEX_FP32, "mixed": _lib.DEX_FP32}[precision]
+        self._ctx = C.c_void_p()
+        check(lib.dex_ctx_create(C.byref(p), C.byref(self._ctx)))
+        info = _lib.DexInfo()
+        check(lib.dex_ctx_info(self._ctx, C.byref(info)))
+        self.n, self.nn, self.m, self.d = info.n, info.nn, info.m, info.d
+        self.delta, self.theta, self.charge = float(delta), float(theta), charge
+        self.precision = precision
+        self.fast_path = bool(info.fast_path)
+        self.device = device
+        self._lib = lib
+
+    # -- lifetime -----------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            self._lib.dex_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- tables ---------------------------------------------------------------------
+    def kematrix(self) -> np.ndarray:
+        t = np.zeros((self.nn, self.nn), order="F")
+        check(self._lib.dex_get_kematrix(self._ctx, _ptr(t)))
+        return t
+
+    def tinv(self) -> np.ndarray:
+        L = 2 * self.nn + 1
+        t = np.zeros(L ** 3)
+        check(self._lib.dex_get_tinv(self._ctx, _ptr(t)))
+        return t.reshape(L, L, L, order="F")  # [i+NN, j+NN, k+NN]
+
+    # -- operator -----------------------------------------------------------------------
+    def mymatmul(self, inp: np.ndarray, out: np.ndarray, howmany: int, checksize: int) -> None:
+        if inp.dtype != np.complex128 or out.dtype != np.complex128:
+            raise TypeError("complex128 arrays required")
+        if inp.size < howmany * checksize or out.size < howmany * checksize:
+            raise DexError(_lib.DEX_ESIZE, "array smaller than howmany*checksize")
+        if not (inp.flags.f_contiguous or inp.flags.c_contiguous) or not (
+                out.flags.f_contiguous or out.flags.c_contiguous):
+            raise ValueError("contiguous arrays required")
+        check(self._lib.dex_matvec(self._ctx, _ptr(inp), _ptr(out), howmany, checksize))
+
+    def hammult(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        if x.size % self.d:
+            raise DexError(_lib.DEX_ESIZE, f"vector length {x.size} is not a multiple of {self.d}")
+        y = np.empty_like(x)
+        self.mymatmul(x, y, x.size // self.d, self.d)
+        return y
+
+    def mydotsub(self, bras: np.ndarray, kets: np.ndarray, size: int, nn: int, mm: int) -> np.ndarray:
+        out = np.zeros((nn, mm), dtype=np.complex128, order="F")
+        b = np.asfortranarray(bras, dtype=np.complex128)
+        k = np.asfortranarray(kets, dtype=np.complex128)
+        check(self._lib.dex_dots(self._ctx, _ptr(b), _ptr(k), size, nn, mm,
+                                 out.ctypes.data_as(_dp)))
+        return out
+
+    # device variants (torch tensors as memory)
+    def matvec_device(self, x, y, howmany: int | None = None, single: bool = False) -> None:
+        howmany = x.numel() // self.d if howmany is None else howmany
+        check(self._lib.dex_matvec_device(self._ctx, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+                                          howmany, self.d, int(single)))
+
+    def dots_device(self, bras, kets, nn: int, mm: int) -> np.ndarray:
+        out = np.zeros((nn, mm), dtype=np.complex128, order="F")
+        check(self._lib.dex_dots_device(self._ctx, C.c_void_p(bras.data_ptr()), C.c_void_p(kets.data_ptr()),
+                                        self.d, nn, mm, out.ctypes.data_as(_dp)))
+        return out
+
+    def fill_gaussian_device(self, x, ncols: int, seed: int, stream_id: int = 0, singlet: bool = True):
+        check(self._lib.dex_fill_gaussian_device(self._ctx, C.c_void_p(x.data_ptr()), ncols, seed,
+                                                 stream_id, int(singlet)))
+
+    def sync(self):
+        check(self._lib.dex_sync(self._ctx))
+
+    def set_kernel_timing(self, enable: bool = True):
+        check(self._lib.dex_set_kernel_timing(self._ctx, int(enable)))
+
+    def kernel_timing(self):
+        a, b = C.c_double(), C.c_double()
+        check(self._lib.dex_get_kernel_timing(self._ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def residuals_device(self, x, e: np.ndarray) -> np.ndarray:
+        nev = e.size
+        ee = np.ascontiguousarray(e, dtype=np.complex128)
+        out = np.zeros(nev)
+        check(self._lib.dex_residuals_device(self._ctx, C.c_void_p(x.data_ptr()), ee.view(np.float64).ctypes.data_as(_dp),
+                                             nev, _ptr(out)))
+        return out
+
+    def footprint(self, nev: int, ns: int = 1) -> int:
+        return int(self._lib.dex_gplmr_footprint(self._ctx, nev, ns))
+
+    # -- eigensolver --------------------------------------------------------------------
+    @staticmethod
+    def _opts(strict: bool, verbose: bool, refresh_every: int, fp64_restarts: int, fp32_tol: float):
+        o = _lib.DexSolveOpts()
+        _lib.load().dex_solve_opts_default(C.byref(o))
+        o.strict_cholesky = int(strict)
+        o.verbose = int(verbose)
+        o.refresh_every = refresh_every
+        o.fp64_restarts = fp64_restarts
+        o.fp32_tol = fp32_tol
+        return o
+
+    def _result(self, st, r, e):
+        return SolveResult(e=e, residuals=None, iterations=r.iterations, converged=bool(r.converged),
+                           matvecs=r.matvecs, max_residual=r.max_residual, t_total=r.t_total,
+                           t_matvec=r.t_matvec, t_gram=r.t_gram, t_update=r.t_update,
+                           t_smalleig=r.t_smalleig, fallbacks=r.fallbacks,
+                           fp64_restarts_used=r.fp64_restarts_used, status=st)
+
+    def gplmr_eig(self, x: np.ndarray, sigma: complex, nev: int, ns: int = 1, maxit: int = 100,
+                  tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5, strict: bool = False,
+                  verbose: bool = False, refresh_every: int = 0, fp64_restarts: int = 2,
+                  fp32_tol: float = 1e-5, m: int | None = None, mb: int | None = None) -> SolveResult:
+        """GPLMR_EIG on a host block ``x`` (D*NEV complex128, vector-major), updated in place."""
+        if x.dtype != np.complex128 or not (x.flags.f_contiguous or x.flags.c_contiguous):
+            raise TypeError("x must be a contiguous complex128 array")
+        m = self.m if m is None else m
+        mb = self.m if mb is None else mb
+        if x.size != m * mb * nev:
+            raise DexError(_lib.DEX_ESIZE, "x.size != M*MB*NEV")
+        e = np.zeros(nev, dtype=np.complex128)
+        r = _lib.DexResult()
+        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol)
+        st = self._lib.dex_gplmr_eig(self._ctx, _ptr(x), e.view(np.float64).ctypes.data_as(_dp), m, mb,
+                                     float(np.real(sigma)), float(np.imag(sigma)), nev, ns, maxit, tol,
+                                     inner_iter, outmodulus, C.byref(o), C.byref(r))
+        check(st, allow=(_lib.DEX_OK, _lib.DEX_ENOTCONVERGED))
+        return self._result(st, r, e)
+
+    def gplmr_eig_device(self, x, sigma: complex, nev: int, ns: int = 1, maxit: int = 100,
+                         tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5,
+                         strict: bool = False, verbose: bool = False, refresh_every: int = 0,
+                         fp64_restarts: int = 2, fp32_tol: float = 1e-5) -> SolveResult:
+        """GPLMR_EIG with the block resident on the device (torch complex128 tensor)."""
+        e = np.zeros(nev, dtype=np.complex128)
+        r = _lib.DexResult()
+        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol)
+        st = self._lib.dex_gplmr_eig_device(self._ctx, C.c_void_p(x.data_ptr()),
+                                            e.view(np.float64).ctypes.data_as(_dp), self.m, self.m,
+                                            float(np.real(sigma)), float(np.imag(sigma)), nev, ns,
+                                            maxit, tol, inner_iter, outmodulus, C.byref(o), C.byref(r))
+        check(st, allow=(_lib.DEX_OK, _lib.DEX_ENOTCONVERGED))
+        return self._result(st, r, e)

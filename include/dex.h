@@ -1,0 +1,180 @@
+/*
+ * dex.h - C-ABI of the MI355X implementation of the helium two-electron
+ * complex-scaled sinc-DVR matvec and the GPLMR / harmonic-Ritz block
+ * eigensolver (one GPU, one HIP stream per context).
+ *
+ * Conventions (those of the reference's Fortran boundary,
+ * complexscale-djhnow/code/doublyexcited.F90:24-59 and gplmr.F90:9-36):
+ *   - complex numbers are interleaved (re, im) doubles, i.e. COMPLEX*16;
+ *   - a two-electron vector is X(P, Q), P = electron 1 flattened with i1
+ *     fastest, Q = electron 2 (hammod.F90:149, initmod.F90:86-98); its length
+ *     is D = NN^6 with NN = 2N+1 points per axis;
+ *   - a block of k vectors is vector-major: vector j starts at j*D;
+ *   - every call returns 0 on success or a DEX_E* status; nothing calls exit().
+ *
+ * Threading: a context is used by one host thread at a time; calls on one
+ * context are serialised on its stream.  Contexts are independent.
+ */
+#ifndef DEX_H
+#define DEX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+  DEX_OK = 0,
+  DEX_ESIZE = 1,          /* checksize / M*MB mismatch: "SIZE ERRORRRR 66", doublyexcited.F90:33-36 */
+  DEX_ECHOLESKY = 2,      /* cholQR failed (strict mode): "Cholesky failed!!!", lamod.f90:169-175 */
+  DEX_EGEVP = 3,          /* projected pencil solve failed, gplmr.F90:430-433 */
+  DEX_ENOTCONVERGED = 4,  /* MAXIT reached with max residual > TOL (silent in the reference, gplmr.F90:86) */
+  DEX_EHIP = 5,           /* a HIP runtime call failed */
+  DEX_EOOM = 6,           /* device allocation failed */
+  DEX_EINVAL = 7,         /* invalid argument */
+  DEX_EADMISSION = 8,     /* the solve does not fit the context's HBM cap */
+  DEX_EUNSUPPORTED = 9,   /* grid size / precision combination not built */
+  DEX_EZEROBLOCK = 10     /* all-zero block in cholQR ("Warning rsum=0"), lamod.f90:157-166 */
+};
+
+enum { DEX_FP64 = 0, DEX_FP32 = 1 };
+
+/* Grid and Hamiltonian parameters: the fields of PARAMETERS / MOLECULE the path
+ * uses (structs.F90:28-60, input.F90:27-44).  One nucleus of charge `charge`
+ * at the origin grid point (NUC_ON_GRID, RNUC = 0: input.F90:244-250). */
+typedef struct dex_params {
+  int32_t n;          /* half-width N; NN = 2N+1 (initmod.F90:31) */
+  double delta;       /* DELTA, grid spacing (bohr) */
+  double theta;       /* THETA, complex scaling angle (rad) */
+  int32_t charge;     /* CHARGE (default 2, input.F90:40) */
+  int32_t nsmall;     /* NSMALL (default 22, input.F90:31) */
+  int32_t tfactor;    /* TFACTOR (default 4, input.F90:33) */
+  int32_t precision;  /* DEX_FP64 or DEX_FP32 arithmetic of the kinetic contraction */
+  int32_t device;     /* HIP device ordinal */
+} dex_params;
+
+typedef struct dex_ctx dex_ctx;
+
+typedef struct dex_info {
+  int32_t n, nn;
+  int64_t m;          /* NN^3 */
+  int64_t d;          /* NN^6 = vector length */
+  double delta, theta;
+  int32_t precision;
+  int32_t fast_path;  /* 1 if the LDS two-pass matvec is used for this NN/precision */
+} dex_info;
+
+/* Fills params with the reference defaults (input.F90:27-44): N=2, DELTA=0.5,
+ * THETA=0, CHARGE=2, NSMALL=22, TFACTOR=4, FP64, device 0. */
+void dex_params_default(dex_params *p);
+
+/* INIT_STRUCTS(NELEC=2) + INIT_HAMILTONIAN (input.F90:365-417,
+ * initmod.F90:174-228): builds T (KEMATRIX), Tinv (GETINVERSE) and the
+ * potential tables, uploads them to the device and creates the stream. */
+int dex_ctx_create(const dex_params *p, dex_ctx **out);
+int dex_ctx_destroy(dex_ctx *ctx);
+int dex_ctx_info(const dex_ctx *ctx, dex_info *info);
+
+/* Copies of the host-side tables (for checking): T(NN,NN) real before the
+ * 1/SCALE^2 factor, Tinv(-NN:NN)^3 real (the reference's HAM%TINV). */
+int dex_get_kematrix(const dex_ctx *ctx, double *t /* NN*NN */);
+int dex_get_tinv(const dex_ctx *ctx, double *tinv /* (2NN+1)^3 */);
+
+/* MYMATMUL(IN,OUT,HOWMANY,CHECKSIZE), doublyexcited.F90:24-44, i.e. HOWMANY
+ * calls of HAMMULT (hammod.F90:19-67).  Host pointers; checksize must equal D. */
+int dex_matvec(dex_ctx *ctx, const double *in, double *out, int32_t howmany, int64_t checksize);
+/* Same on device pointers (complex128, or complex64 when the context is
+ * DEX_FP32 and `single` is 1), ordered on the context's stream. */
+int dex_matvec_device(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
+                      int64_t checksize, int32_t single);
+
+/* MYDOTSUB(BRAS,KETS,SIZE,NN,MM,OUTDOTS), doublyexcited.F90:47-59:
+ * OUT(NN,MM) = BRAS^H KETS, column-major.  Host or device pointers. */
+int dex_dots(dex_ctx *ctx, const double *bras, const double *kets, int64_t size,
+             int32_t nn, int32_t mm, double *out);
+int dex_dots_device(dex_ctx *ctx, const void *bras, const void *kets, int64_t size,
+                    int32_t nn, int32_t mm, double *out_host);
+
+typedef struct dex_result {
+  int32_t iterations;     /* ITER - 1 at exit */
+  int32_t converged;      /* 1 if max residual <= TOL */
+  int64_t matvecs;        /* HAMMULT applications */
+  double max_residual;
+  double t_total;         /* TOT_TIME (s) */
+  double t_matvec;        /* TSPMV (s), gplmr.F90:71-74 */
+  double t_prec;          /* TPREC (s) */
+  double t_gram;          /* projection + orthogonalisation Grams (s) */
+  double t_update;        /* block recombinations (s) */
+  double t_smalleig;      /* host projected pencil (s) */
+  int32_t fallbacks;      /* orthonormalisations that left the plain cholQR path */
+  int32_t fp64_restarts_used; /* FP32 contexts: FP64 restarts that ran */
+} dex_result;
+
+typedef struct dex_solve_opts {
+  int32_t strict_cholesky; /* 1: fail with DEX_ECHOLESKY where the reference STOPs */
+  int32_t verbose;         /* print "EIG j = ..." every OUTMODULUS iterations like gplmr.F90:343-352 */
+  int32_t refresh_every;   /* >0: recompute AX = H X every this many iterations (0 = never, as the reference) */
+  int32_t fp64_restarts;   /* FP32 contexts: at most this many FP64 restarts after the FP32 phase */
+  double fp32_tol;         /* FP32 contexts: residual target of the FP32-matvec phase */
+} dex_solve_opts;
+
+void dex_solve_opts_default(dex_solve_opts *o);
+
+/* GPLMR_EIG(HAM,PARAMS,MPIDATA,GMSOLVER,X,E,M,MB,SIGMAIN,NEV,NS,MAXIT,TOL,
+ *           PRECONDITIONER,OUTMODULUS), gplmr.F90:9.
+ * X (M*MB*NEV complex, host): start block in, eigenvectors out.  E (NEV
+ * complex, host): Rayleigh quotients out.  M*MB must equal D (one process
+ * holds the whole vector).  inner_iter is GM_MAX_INNER_ITER: 0 (no inner
+ * solve) or 1 (W <- H W), gplmr.F90:466-502.  res/opts may be NULL. */
+int dex_gplmr_eig(dex_ctx *ctx, double *x, double *e, int32_t m, int32_t mb,
+                  double sigma_re, double sigma_im, int32_t nev, int32_t ns,
+                  int32_t maxit, double tol, int32_t inner_iter, int32_t outmodulus,
+                  const dex_solve_opts *opts, dex_result *res);
+/* Same with X resident on the device (complex128, M*MB*NEV). */
+int dex_gplmr_eig_device(dex_ctx *ctx, void *x_dev, double *e, int32_t m, int32_t mb,
+                         double sigma_re, double sigma_im, int32_t nev, int32_t ns,
+                         int32_t maxit, double tol, int32_t inner_iter, int32_t outmodulus,
+                         const dex_solve_opts *opts, dex_result *res);
+
+/* Seeded Philox4x32-10 complex Gaussian start block on the device: entry i of
+ * column c is (g1 + i g2)/sqrt(2) from counter (i, c, stream_id, 0) and key
+ * seed; optionally singlet-projected, X <- (X + X^T)/2 per column. */
+int dex_fill_gaussian_device(dex_ctx *ctx, void *x_dev, int32_t ncols, uint64_t seed,
+                             uint32_t stream_id, int32_t singlet);
+int dex_sync(dex_ctx *ctx);
+
+/* Residual norms ||H x_j - E_j x_j|| / ||x_j|| recomputed in FP64 from scratch. */
+int dex_residuals_device(dex_ctx *ctx, const void *x_dev, const double *e, int32_t nev,
+                         double *res_out);
+
+/* Bytes of device memory dex_gplmr_eig needs for (nev, ns) on this context. */
+int64_t dex_gplmr_footprint(const dex_ctx *ctx, int32_t nev, int32_t ns);
+
+/* Host-only set-up and small-matrix routines (no device use), exported so the
+ * host arithmetic can be checked on machines without a GPU:
+ *   KEMATRIX (hammod.F90:235-260): t = NN x NN column-major, real;
+ *   GETINVERSE (tooth.f90:14-101): tinv = (2 nout + 1)^3 reals, first index fastest;
+ *   SOLVE_GEVP (gplmr.F90:395-439): the NEV pairs of K y = mu M y with smallest
+ *   |mu|; kp, mp: k x k complex (leading dimension ld); coords: k x nev complex
+ *   (unit 2-norm columns), mu: nev complex. */
+int dex_host_kematrix(int32_t nn, double delta, double *t);
+int dex_host_getinverse(int32_t nout, double delta, int32_t nsmall, int32_t tfactor, double *tinv);
+int dex_host_solve_gevp(int32_t k, const double *kp, const double *mp, int32_t ld, int32_t nev,
+                        double *coords, double *mu);
+
+/* Human-readable text of a status code, and the last error message on this thread. */
+const char *dex_status_string(int status);
+const char *dex_last_error(void);
+
+/* Device event timing of the dominant matvec kernels of the most recent
+ * dex_matvec_device call with timing enabled (ms per launch, -1 if none). */
+int dex_set_kernel_timing(dex_ctx *ctx, int32_t enable);
+int dex_get_kernel_timing(const dex_ctx *ctx, double *pass1_ms, double *pass2_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DEX_H */

@@ -1,0 +1,247 @@
+"""CPU oracle (test infrastructure only) for the GPLMR / harmonic-Ritz block eigensolver.
+
+Imported only by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py``; the product never routes through it.
+
+A numpy restatement of ``GPLMR_EIG`` (complexscale-djhnow/code/gplmr.F90:9-386)
+and its dense linear-algebra helpers (lamod.f90:21-180), with the projected
+pencil solved by LAPACK ZGGEV through scipy exactly as ``SOLVE_GEVP`` does
+(gplmr.F90:395-439).  The operator is a callback ``amul(block) -> block`` on
+(D, k) complex128 blocks, the ``MATBLOCK`` convention (gplmr.F90:443-462).
+
+Pinning: the fixed point is checked against an independent dense
+eigendecomposition (numpy ``eig``) at tiny grids, the lamod identities
+(Q^H Q = I, R upper triangular) are checked directly, and the solver is driven to
+the reference's logged singlet resonance at NN=11, delta=2, theta=0.2.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Callable
+
+import numpy as np
+import scipy.linalg as sla
+
+
+class GplmrError(RuntimeError):
+    """Raised where the reference prints and STOPs."""
+
+
+def normalize(x):
+    """lamod.f90:21-48: unit 2-norm columns."""
+    nrm = np.sqrt(np.einsum("ij,ij->j", x.real, x.real) + np.einsum("ij,ij->j", x.imag, x.imag))
+    x /= nrm[None, :]
+    return x
+
+
+def crossmult_diag(x, y):
+    """lamod.f90:51-76: D = diag(X^H Y)."""
+    return np.einsum("ij,ij->j", x.conj(), y)
+
+
+def colnorms(x):
+    """lamod.f90:79-104: sqrt(diag(X^H X))."""
+    return np.sqrt(np.einsum("ij,ij->j", x.real, x.real) + np.einsum("ij,ij->j", x.imag, x.imag))
+
+
+def crossmult(x, y):
+    """lamod.f90:108-125: Z = X^H Y (ZGEMM 'C','N')."""
+    return x.conj().T @ y
+
+
+STRICT_CHOLESKY = False  # True: fail where the reference STOPs (lamod.f90:169-175)
+PIVOT_TOL = 1e-12  # smallest accepted pivot of the unit-diagonal Gram
+
+
+def _upper_chol(g):
+    return np.linalg.cholesky(g).conj().T  # upper factor: G = R^H R
+
+
+def orth_block(x, drop, strict=None):
+    """Orthonormalise the columns of ``x``: cholQR (lamod.f90:128-180) plus a fallback.
+
+    Returns ``(xq, t)`` with ``xq = x @ t``.  Where ZPOTRF succeeds and the
+    unit-diagonal Gram has all pivots above ``PIVOT_TOL``, ``t = R^{-1}`` with
+    ``R`` the reference's upper Cholesky factor, exactly as cholQR.  Where the
+    reference would STOP ("Cholesky failed") or continue with a numerically
+    singular R, this restatement (unless ``strict``) takes the eigendecomposition
+    of the unit-diagonal Gram (SVQB, Stathopoulos & Wu, SIAM J. Sci. Comput. 23
+    (2002) 2165) and, when ``drop``, keeps only the independent directions
+    (fewer columns), otherwise clamps the small eigenvalues (same column count);
+    one more cholQR pass then restores orthonormality.
+    """
+    strict = STRICT_CHOLESKY if strict is None else strict
+    g = crossmult(x, x)
+    if np.sum(np.abs(g)) == 0.0:
+        raise GplmrError("zero block in cholQR")  # lamod.f90:157-166
+    dg = np.sqrt(np.maximum(np.real(np.diag(g)), 0.0))
+    ok = bool(np.all(dg > 0))
+    if ok:
+        try:
+            r = _upper_chol(g)
+            piv = (np.real(np.diag(r)) / dg) ** 2
+            ok = bool(np.all(piv > PIVOT_TOL))
+        except np.linalg.LinAlgError:
+            ok = False
+    if ok:
+        t = sla.solve_triangular(r, np.eye(len(dg), dtype=np.complex128), lower=False)
+        return x @ t, t
+    if strict:
+        raise GplmrError("Cholesky failed")  # lamod.f90:169-175
+    k = g.shape[0]
+    dinv = np.where(dg > 0, 1.0 / np.where(dg > 0, dg, 1.0), 0.0)
+    gs = dinv[:, None] * g * dinv[None, :]
+    lam, u = np.linalg.eigh(gs)
+    lmax = lam.max()
+    if drop:
+        keep = lam > PIVOT_TOL * lmax
+        t = dinv[:, None] * (u[:, keep] / np.sqrt(lam[keep])[None, :])
+    else:
+        lamc = np.maximum(lam, PIVOT_TOL * lmax)
+        t = dinv[:, None] * (u / np.sqrt(lamc)[None, :])
+    xq = x @ t
+    g2 = crossmult(xq, xq)
+    r2 = _upper_chol(g2)
+    t2 = sla.solve_triangular(r2, np.eye(g2.shape[0], dtype=np.complex128), lower=False)
+    return xq @ t2, t @ t2
+
+
+def solve_gevp(kp, mp, k, nev):
+    """gplmr.F90:395-439: ZGGEV of the leading k x k pencil, NEV smallest |alpha/beta|.
+
+    EIGSORT (gplmr.F90:640-660) picks the first minimum on ties (MINLOC).
+    Returns (vectors (k, nev), mu (nev)).
+    """
+    a = kp[:k, :k]
+    b = mp[:k, :k]
+    try:
+        w, vr = sla.eig(a, b, right=True, homogeneous_eigvals=True)
+    except (np.linalg.LinAlgError, ValueError) as exc:
+        raise GplmrError("SOLVE OF THE PROJECTED EIGENVALUE PROBLEM FAILED") from exc
+    alpha, beta = w
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lam = np.abs(alpha / beta)
+    lam = np.where(np.isnan(lam), np.inf, lam)
+    idx = np.argsort(lam, kind="stable")[:nev]
+    return vr[:, idx], (alpha / beta)[idx]
+
+
+@dataclasses.dataclass
+class GplmrResult:
+    x: np.ndarray
+    e: np.ndarray
+    resnrm: np.ndarray
+    iters: int
+    converged: bool
+    matvecs: int
+    history: list
+
+
+def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: complex,
+              nev: int, ns: int = 1, maxit: int = 100, tol: float = 1e-8,
+              inner_iter: int = 0, callback=None) -> GplmrResult:
+    """GPLMR_EIG, gplmr.F90:9-386, with GM_MAX_INNER_ITER in {0, 1}.
+
+    ``inner_iter == 1`` applies PRECBLOCK's ``W <- H W`` (gplmr.F90:491-493);
+    ``0`` skips the preconditioner (guards at gplmr.F90:92,118,151).
+    """
+    if ns < 1:
+        raise GplmrError("NS must be >= 1")  # R is (NS*NEV, NEV), gplmr.F90:36
+    if inner_iter not in (0, 1):
+        raise GplmrError("only GM_MAX_INNER_ITER 0 or 1 is restated")
+    x = np.array(x0, dtype=np.complex128, copy=True)
+    dim = x.shape[0]
+    ldr = ns * nev
+    nproj = (3 + ns) * nev
+    nmat = 0
+
+    def matblock(b):
+        nonlocal nmat
+        nmat += b.shape[1]
+        return amul(b)
+
+    def prec(b):
+        return matblock(b) if inner_iter == 1 else b
+
+    normalize(x)
+    ax = matblock(x)
+    e = crossmult_diag(x, ax)
+    w = ax - x * e[None, :]
+    resnrm = colnorms(w)
+    maxres = resnrm.max()
+    p = np.zeros((dim, 0), np.complex128)
+    ap = np.zeros_like(p)
+    it = 1
+    history = []
+    while it <= maxit and maxres > tol:
+        x, t = orth_block(x, drop=False)  # :88
+        ax = ax @ t  # :90
+        w = prec(w)
+        w = w - x @ crossmult(x, w)  # :101-103
+        w, _ = orth_block(w, drop=True)  # :105
+        aw = matblock(w)  # :108
+        # S_1 = AW - W E  (:114-116); E pairs with the leading columns
+        ew = e[:w.shape[1]]
+        sblocks, asblocks = [], []
+        snew = aw - w * ew[None, :]
+        for l in range(1, ns + 1):  # :114-172
+            if l > 1:
+                prev, aprev = sblocks[-1], asblocks[-1]
+                snew = aprev - prev * e[:prev.shape[1]][None, :]
+            snew = prec(snew)
+            snew = snew - x @ crossmult(x, snew)  # :127-128
+            snew = snew - w @ crossmult(w, snew)  # :129-130
+            if sblocks:
+                sall = np.concatenate(sblocks, axis=1)
+                snew = snew - sall @ crossmult(sall, snew)
+            snew, _ = orth_block(snew, drop=True)  # :131
+            sblocks.append(snew)
+            asblocks.append(matblock(snew))  # :134
+        s = np.concatenate(sblocks, axis=1)
+        as_ = np.concatenate(asblocks, axis=1)
+        if it > 1 and p.shape[1] > 0:  # :177-196
+            r = crossmult(x, p)
+            p = p - x @ r
+            ap = ap - ax @ r
+            r = crossmult(w, p)
+            p = p - w @ r
+            ap = ap - aw @ r
+            r = crossmult(s, p)
+            p = p - s @ r
+            ap = ap - as_ @ r
+            p, t = orth_block(p, drop=True)
+            ap = ap @ t
+        # harmonic projection (:201-270)
+        q = np.concatenate([x, w, s, p], axis=1)
+        aq = np.concatenate([ax, aw, as_, ap], axis=1)
+        asig = aq - sigma * q
+        kp = asig.conj().T @ asig
+        mp = asig.conj().T @ q
+        k = q.shape[1]  # iteration 1 has no P block (:286-290)
+        coords, mu = solve_gevp(kp, mp, k, nev)
+        nw, nsb = w.shape[1], s.shape[1]
+        cx = coords[:nev]
+        cw = coords[nev:nev + nw]
+        cs = coords[nev + nw:nev + nw + nsb]
+        cp = coords[nev + nw + nsb:]
+        p_new = s @ cs + w @ cw  # :307-312
+        ap_new = as_ @ cs + aw @ cw
+        if p.shape[1] > 0:  # :298-305
+            p_new += p @ cp
+            ap_new += ap @ cp
+        p, ap = p_new, ap_new
+        x = x @ cx + p  # :316-318
+        ax = ax @ cx + ap
+        nrm = colnorms(x)  # :324
+        x /= nrm[None, :]
+        ax /= nrm[None, :]
+        e = crossmult_diag(x, ax)  # :332
+        w = ax - x * e[None, :]  # :334-336
+        resnrm = colnorms(w)
+        maxres = resnrm.max()
+        history.append((it, e.copy(), resnrm.copy()))
+        if callback is not None:
+            callback(it, e, resnrm)
+        it += 1
+    return GplmrResult(x=x, e=e, resnrm=resnrm, iters=it - 1, converged=bool(maxres <= tol),
+                       matvecs=nmat, history=history)

@@ -1,0 +1,90 @@
+"""CPU checks of the native library: it loads, exports every symbol the header
+declares, and its host-side arithmetic (KEMATRIX, GETINVERSE, the projected
+pencil) agrees with the oracle and with LAPACK.  No device calls."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from doublyexcited_amd import _lib
+from oracle.hamiltonian import getinverse, kematrix
+
+ROOT = Path(__file__).resolve().parent.parent
+_dp = C.POINTER(C.c_double)
+
+
+def header_symbols():
+    text = (ROOT / "include" / "dex.h").read_text()
+    return sorted(set(re.findall(r"\b(dex_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(_lib.EXPORTS) == syms
+
+
+def test_status_strings_and_defaults():
+    lib = _lib.load()
+    assert lib.dex_status_string(_lib.DEX_ESIZE).decode() == "size mismatch"
+    p = _lib.DexParams()
+    lib.dex_params_default(C.byref(p))
+    assert (p.n, p.delta, p.charge, p.nsmall, p.tfactor) == (2, 0.5, 2, 22, 4)
+    o = _lib.DexSolveOpts()
+    lib.dex_solve_opts_default(C.byref(o))
+    assert o.fp64_restarts == 2 and o.strict_cholesky == 0
+
+
+def test_host_kematrix_matches_oracle():
+    lib = _lib.load()
+    t = np.zeros(21 * 21)
+    assert lib.dex_host_kematrix(21, 0.4, t.ctypes.data_as(_dp)) == 0
+    np.testing.assert_allclose(t.reshape(21, 21, order="F"), kematrix(21, 0.4), rtol=1e-15)
+
+
+@pytest.mark.parametrize("nout,delta", [(11, 2.0), (21, 0.4), (25, 0.35)])
+def test_host_getinverse_matches_oracle(nout, delta):
+    lib = _lib.load()
+    L = 2 * nout + 1
+    t = np.zeros(L ** 3)
+    assert lib.dex_host_getinverse(nout, delta, 22, 4, t.ctypes.data_as(_dp)) == 0
+    ref = getinverse(nout, delta)
+    np.testing.assert_allclose(t.reshape(L, L, L, order="F"), ref, rtol=1e-12, atol=1e-14)
+
+
+def test_host_getinverse_rejects_small_big_grid():
+    lib = _lib.load()
+    t = np.zeros(1)
+    assert lib.dex_host_getinverse(100, 0.5, 22, 4, t.ctypes.data_as(_dp)) == _lib.DEX_EINVAL
+
+
+@pytest.mark.parametrize("k,nev,seed", [(8, 2, 0), (32, 8, 1), (64, 16, 2), (24, 8, 3)])
+def test_host_pencil_matches_lapack_zggev(k, nev, seed):
+    lib = _lib.load()
+    rng = np.random.default_rng(seed)
+    n = 400
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    q, _ = np.linalg.qr(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))
+    sigma = 0.3 - 0.1j
+    b = a @ q - sigma * q
+    kp = np.asfortranarray(b.conj().T @ b)
+    mp = np.asfortranarray(b.conj().T @ q)
+    coords = np.zeros((k, nev), dtype=np.complex128, order="F")
+    mu = np.zeros(nev, dtype=np.complex128)
+    st = lib.dex_host_solve_gevp(k, kp.ctypes.data_as(_dp), mp.ctypes.data_as(_dp), k, nev,
+                                 coords.ctypes.data_as(_dp), mu.ctypes.data_as(_dp))
+    assert st == 0
+    w = sla.eig(kp, mp, right=False)
+    ref = w[np.argsort(np.abs(w))[:nev]]
+    np.testing.assert_allclose(np.sort(np.abs(mu)), np.sort(np.abs(ref)), rtol=1e-9)
+    for j in range(nev):
+        y = coords[:, j]
+        assert np.linalg.norm(y) == pytest.approx(1.0, rel=1e-12)
+        r = kp @ y - mu[j] * (mp @ y)
+        assert np.linalg.norm(r) < 1e-9 * np.linalg.norm(kp, 2)

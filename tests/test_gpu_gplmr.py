@@ -1,0 +1,130 @@
+"""GPU parity of the GPLMR / harmonic-Ritz block eigensolver (GPLMR_EIG).
+
+Oracles: a dense eigendecomposition at NN=3, the numpy GPLMR restatement
+(oracle/gplmr.py) from the same start block at NN=5 and NN=7, and the
+reference's logged singlet resonance at NN=11, delta=2, theta=0.2
+(CSCALE-WORK/Outs/Out.S.N5:147).  Tolerances: |dE| <= 1e-9 au for FP64 solves
+converged to residual 1e-10, <= 1e-7 au for the mixed (FP32 matvec + FP64
+restarts) form.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from oracle.gplmr import gplmr_eig
+from oracle.hamiltonian import Hamiltonian, exchange, singlet_project
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dex():
+    import doublyexcited_amd as d
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return d
+
+
+def start_block(h, nev, seed):
+    rng = np.random.default_rng(seed)
+    x0 = rng.standard_normal((h.d, nev)) + 1j * rng.standard_normal((h.d, nev))
+    for j in range(nev):
+        x0[:, j] = singlet_project(x0[:, j], h.m)
+    return np.asfortranarray(x0)
+
+
+def match(a, b):
+    """max over a of the distance to the nearest element of b."""
+    return max(np.min(np.abs(np.asarray(b) - z)) for z in a)
+
+
+def test_gplmr_matches_dense_nn3(dex):
+    print("case: gplmr dense NN=3", flush=True)
+    h = Hamiltonian(1, 0.6, 0.3)
+    hd = h.dense()
+    w, v = np.linalg.eig(hd)
+    sym = np.array([np.linalg.norm(v[:, j] - exchange(v[:, j], h.m)) for j in range(v.shape[1])])
+    ws = w[sym < 1e-6]
+    sigma, nev = 0.5 - 0.1j, 3
+    x = start_block(h, nev, 11)
+    with dex.HeliumGrid(1, 0.6, 0.3) as g:
+        r = g.gplmr_eig(x, sigma, nev, ns=1, maxit=300, tol=1e-10)
+    assert r.converged
+    near = ws[np.argsort(np.abs(ws - sigma))[:nev]]
+    np.testing.assert_allclose(np.sort_complex(r.e), np.sort_complex(near), atol=1e-9)
+    # returned vectors are eigenvectors
+    for j in range(nev):
+        xj = x[:, j]
+        assert np.linalg.norm(hd @ xj - r.e[j] * xj) / np.linalg.norm(xj) < 1e-9
+
+
+@pytest.mark.parametrize("n,nev,sigma", [(2, 4, -0.78 - 0.002j), (3, 4, -1.2 - 0.9j)])
+def test_gplmr_matches_oracle_solver(dex, n, nev, sigma):
+    print(f"case: gplmr vs oracle N={n}", flush=True)
+    h = Hamiltonian(n, 0.5, 0.3)
+    v = h.potential2()
+    x0 = start_block(h, nev, 100 + n)
+    ref = gplmr_eig(lambda b: np.stack([h.hammult(b[:, j], v) for j in range(b.shape[1])], axis=1),
+                    x0, sigma, nev, ns=1, maxit=600, tol=1e-10)
+    assert ref.converged
+    x = x0.copy(order="F")
+    with dex.HeliumGrid(n, 0.5, 0.3) as g:
+        r = g.gplmr_eig(x, sigma, nev, ns=1, maxit=600, tol=1e-10)
+    assert r.converged
+    assert match(r.e, ref.e) < 1e-9 and match(ref.e, r.e) < 1e-9
+    # the same number of iterations to within a few (FP reordering only)
+    assert abs(r.iterations - ref.iters) <= max(3, ref.iters // 10)
+
+
+def test_inner_iter_one_matches_oracle(dex):
+    print("case: gplmr inner=1 N=2", flush=True)
+    h = Hamiltonian(2, 0.5, 0.3)
+    v = h.potential2()
+    nev, sigma = 2, -0.78 - 0.002j
+    x0 = start_block(h, nev, 7)
+    ref = gplmr_eig(lambda b: np.stack([h.hammult(b[:, j], v) for j in range(b.shape[1])], axis=1),
+                    x0, sigma, nev, ns=1, maxit=400, tol=1e-10, inner_iter=1)
+    x = x0.copy(order="F")
+    with dex.HeliumGrid(2, 0.5, 0.3) as g:
+        r = g.gplmr_eig(x, sigma, nev, ns=1, maxit=400, tol=1e-10, inner_iter=1)
+    assert r.converged and ref.converged
+    assert match(r.e, ref.e) < 1e-9
+
+
+def test_mixed_precision_within_1e7(dex):
+    print("case: mixed N=3", flush=True)
+    h = Hamiltonian(3, 0.5, 0.3)
+    nev, sigma = 4, -1.2 - 0.9j
+    x0 = start_block(h, nev, 5)
+    with dex.HeliumGrid(3, 0.5, 0.3) as g:
+        x = x0.copy(order="F")
+        r64 = g.gplmr_eig(x, sigma, nev, ns=1, maxit=800, tol=1e-9)
+    with dex.HeliumGrid(3, 0.5, 0.3, precision="mixed") as g:
+        x = x0.copy(order="F")
+        r32 = g.gplmr_eig(x, sigma, nev, ns=1, maxit=800, tol=1e-9, fp32_tol=1e-5)
+    assert r64.converged and r32.converged
+    assert match(r32.e, r64.e) <= 1e-7
+
+
+def test_reference_logged_singlet_resonance(dex, golden_dir):
+    """NN=11, delta=2, theta=0.2: the reference printed E = -0.713154142874 +
+    0.035227459486 i at residual 7.99e-8 (Out.S.N5:147).  A singlet start with
+    the log's chosen shift (Out.S.N5:126) must reach it."""
+    print("case: logged resonance NN=11", flush=True)
+    g = json.loads((golden_dir / "heplus_nn11_d2_t0.2.json").read_text())
+    ref = complex(*g["singlet_resonance"])
+    p = g["params"]
+    with dex.HeliumGrid(p["N"], p["delta"], p["theta"]) as grid:
+        x = torch.empty(grid.d, dtype=torch.complex128, device="cuda")
+        grid.fill_gaussian_device(x, 1, seed=2024, singlet=True)
+        r = grid.gplmr_eig_device(x, complex(-0.7110341829, 0.0313569500), 1, ns=1, maxit=4000,
+                                  tol=1e-9)
+        res = grid.residuals_device(x, r.e)
+    assert r.converged, (r.iterations, r.max_residual)
+    assert res[0] < 2e-9
+    # the log's own value carries its residual 8e-8 (and two runs differ by 1.3e-9)
+    assert abs(r.e[0] - ref) < 5e-8

@@ -1,0 +1,145 @@
+"""GPU parity of the matvec (HAMMULT / MYMATMUL) and the dots (MYDOTSUB).
+
+Oracle: ``oracle.hamiltonian`` (numpy restatement of hammod.F90:19-67,138-187,
+502-582 and tooth.f90:14-101), itself pinned by tests/test_oracle.py.
+Tolerances: FP64 results within 1e-13 of max|y| (a different summation order
+of the same products); complex64 arithmetic within 2e-6 of max|y|.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.hamiltonian import Hamiltonian, exchange, getinverse
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dex():
+    import doublyexcited_amd as d
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return d
+
+
+def _rand(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def test_matvec_matches_dense_nn3(dex):
+    print("case: dense NN=3", flush=True)
+    h = Hamiltonian(1, 0.6, 0.3)
+    x = _rand(h.d, 1)
+    ref = h.dense() @ x
+    with dex.HeliumGrid(1, 0.6, 0.3) as g:
+        y = g.hammult(x)
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("n,delta,theta", [(2, 0.45, 0.35), (3, 0.5, 0.3), (5, 2.0, 0.2), (6, 0.5, 0.25)])
+def test_matvec_matches_oracle(dex, n, delta, theta):
+    print(f"case: oracle N={n}", flush=True)
+    h = Hamiltonian(n, delta, theta)
+    x = _rand(h.d, n)
+    ref = h.hammult(x)
+    with dex.HeliumGrid(n, delta, theta) as g:
+        assert g.fast_path
+        y = g.hammult(x)
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_block_matvec_howmany(dex):
+    print("case: howmany=3 NN=7", flush=True)
+    h = Hamiltonian(3, 0.5, 0.3)
+    x = np.stack([_rand(h.d, 10 + j) for j in range(3)], axis=1)
+    with dex.HeliumGrid(3, 0.5, 0.3) as g:
+        y = np.empty_like(x, order="F")
+        xf = np.asfortranarray(x)
+        g.mymatmul(xf, y, 3, g.d)
+    for j in range(3):
+        ref = h.hammult(x[:, j])
+        assert np.max(np.abs(y[:, j] - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_checksize_mismatch_is_size_error(dex):
+    with dex.HeliumGrid(2, 0.5, 0.3) as g:
+        x = np.zeros(g.d, np.complex128)
+        y = np.zeros(g.d, np.complex128)
+        with pytest.raises(dex.DexError) as ei:
+            g.mymatmul(x, y, 1, g.d - 1)
+        assert ei.value.status == 1
+
+
+def test_tables_match_oracle(dex):
+    with dex.HeliumGrid(10, 0.4, 0.35) as g:
+        np.testing.assert_allclose(g.tinv(), getinverse(21, 0.4), rtol=1e-12, atol=1e-14)
+
+
+def test_fp32_matvec_matches_oracle(dex):
+    print("case: fp32 NN=11", flush=True)
+    h = Hamiltonian(5, 0.5, 0.3)
+    x = _rand(h.d, 3)
+    ref = h.hammult(x)
+    with dex.HeliumGrid(5, 0.5, 0.3, precision="fp32") as g:
+        y = g.hammult(x)
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    assert err < 2e-6
+    assert err > 1e-12  # the FP32 kernels really ran
+
+
+def _fast_vs_generic(n, delta, theta, seed):
+    """Run the generic six-pass kernels in a child process (DEX_MATVEC=generic)."""
+    code = (
+        "import numpy as np, sys, doublyexcited_amd as d\n"
+        f"g = d.HeliumGrid({n}, {delta}, {theta})\n"
+        "assert not g.fast_path\n"
+        f"rng = np.random.default_rng({seed})\n"
+        "x = rng.standard_normal(g.d) + 1j * rng.standard_normal(g.d)\n"
+        "np.save(sys.argv[1], g.hammult(x))\n"
+    )
+    return code
+
+
+@pytest.mark.parametrize("n,delta,theta", [(8, 0.5, 0.3), (10, 0.4, 0.35)])
+def test_fast_kernels_match_generic_kernels(dex, tmp_path, n, delta, theta):
+    print(f"case: fast vs generic N={n}", flush=True)
+    out = tmp_path / "gen.npy"
+    env = dict(os.environ, DEX_MATVEC="generic")
+    subprocess.run([sys.executable, "-c", _fast_vs_generic(n, delta, theta, 5), str(out)], env=env,
+                   check=True, timeout=300)
+    ref = np.load(out)
+    with dex.HeliumGrid(n, delta, theta) as g:
+        assert g.fast_path
+        rng = np.random.default_rng(5)
+        x = rng.standard_normal(g.d) + 1j * rng.standard_normal(g.d)
+        y = g.hammult(x)
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_exchange_symmetry_nn21(dex):
+    print("case: exchange NN=21", flush=True)
+    with dex.HeliumGrid(10, 0.4, 0.35) as g:
+        x = torch.randn(g.d, dtype=torch.complex128, device="cuda")
+        y = torch.empty_like(x)
+        g.matvec_device(x, y)
+        xp = x.view(g.m, g.m).t().contiguous().view(-1)
+        yp = torch.empty_like(x)
+        g.matvec_device(xp, yp)
+        g.sync()
+        d = (yp.view(g.m, g.m).t().contiguous().view(-1) - y).abs().max().item()
+        assert d <= 1e-13 * y.abs().max().item()
+
+
+def test_dots_match_numpy(dex):
+    with dex.HeliumGrid(3, 0.5, 0.3) as g:
+        b = np.asfortranarray(np.stack([_rand(g.d, 20 + j) for j in range(3)], axis=1))
+        k = np.asfortranarray(np.stack([_rand(g.d, 30 + j) for j in range(5)], axis=1))
+        out = g.mydotsub(b, k, g.d, 3, 5)
+    np.testing.assert_allclose(out, b.conj().T @ k, rtol=1e-12, atol=1e-9)

@@ -1,0 +1,156 @@
+"""CPU checks that pin the oracle before anything is compared against it.
+
+* KEMATRIX closed form vs the reference's sineDVR formula (SineDVR.F90:29-37).
+* GETINVERSE: the defining identity T3D Tinv = 2 pi delta_0 on the small grid,
+  delta^2 scaling, the far field, and (through the dense He+ Hamiltonian)
+  the reference's logged eigenvalues.
+* HAMMULT: the einsum restatement against an independently built Kronecker
+  Hamiltonian, exchange symmetry and complex symmetry.
+* GPLMR: the fixed point against a dense eigendecomposition.
+"""
+import json
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle.hamiltonian import Hamiltonian, exchange, getinverse, kematrix, singlet_project
+from oracle.gplmr import gplmr_eig, orth_block
+
+
+def test_kematrix_matches_sinedvr_formula():
+    # sineDVR: ketot(i) = [pi^2/3 (i=0) | 2 (-1)^i / i^2] / 2 / spacing^2, kematrix(i,j) = ketot(i-j)
+    nn, d = 9, 0.37
+    t = kematrix(nn, d)
+    for i in range(nn):
+        for j in range(nn):
+            m = i - j
+            ref = (np.pi ** 2 / 3.0 if m == 0 else 2.0 / m ** 2 * (-1) ** m) / 2.0 / d ** 2
+            assert t[i, j] == pytest.approx(ref, rel=1e-15)
+
+
+def test_kematrix_plane_wave_symbol():
+    # the infinite sinc-DVR kinetic matrix has symbol k^2/2 for |k| < pi/delta:
+    # sum_m tau(m) e^{i k m delta} = (k delta)^2 / 2, checked on a long grid row
+    d = 0.5
+    n = 40001
+    c = n // 2
+    t = kematrix(n, d)[c]
+    x = (np.arange(n) - c) * d
+    for k in (0.3, 1.1, 2.7, 5.0):
+        val = np.sum(t * np.cos(k * x))
+        assert val == pytest.approx(k * k / 2.0, abs=2e-4)
+
+
+@pytest.fixture(scope="module")
+def tinv_full_d07():
+    # whole big grid (nout = nbig), delta = 0.7, small nsmall to keep it quick
+    return getinverse(nout=32, delta=0.7, nsmall=8, tfactor=4)
+
+
+def test_tinv_solves_coulomb_poisson_on_small_grid(tinv_full_d07):
+    nsmall, nbig, d = 8, 32, 0.7
+    tv = tinv_full_d07  # indexed [i+nbig, j+nbig, k+nbig]
+    big = np.arange(-nbig, nbig + 1)
+    small = np.arange(-nsmall, nsmall + 1)
+    dd = small[:, None] - big[None, :]
+    tsb = np.where(dd == 0, np.pi ** 2 / (6 * d * d),
+                   np.where(np.abs(dd) % 2 == 0, 1.0, -1.0) / (d * d * np.where(dd == 0, 1, dd) ** 2))
+    s = slice(nbig - nsmall, nbig + nsmall + 1)
+    lap = (np.einsum("ip,pjk->ijk", tsb, tv[:, s, s]) + np.einsum("jp,ipk->ijk", tsb, tv[s, :, s])
+           + np.einsum("kp,ijp->ijk", tsb, tv[s, s, :]))
+    target = np.zeros_like(lap)
+    target[nsmall, nsmall, nsmall] = 2 * np.pi
+    assert np.max(np.abs(lap - target)) < 1e-11
+
+
+def test_tinv_far_field_and_scaling():
+    n = 24
+    t1 = getinverse(n, 0.5)
+    t2 = getinverse(n, 0.8)
+    # delta^2 scaling of the whole table
+    np.testing.assert_allclose(t1 / 0.25, t2 / 0.64, rtol=1e-11, atol=1e-13)
+    # outside [-22, 22]^3 the table is the bare Coulomb value delta^2/|n|
+    idx = np.array([23, 0, 1]) + n
+    assert t1[tuple(idx)] == pytest.approx(0.25 / np.sqrt(23 ** 2 + 1), rel=1e-15)
+    assert t1[n, n, n] > 0.25  # the tooth: finite, larger than the neighbouring Coulomb values
+
+
+def test_heplus_eigenvalues_match_reference_log(golden_dir):
+    g = json.loads((golden_dir / "heplus_nn11_d2_t0.2.json").read_text())
+    p = g["params"]
+    h = Hamiltonian(p["N"], p["delta"], p["theta"], p["charge"])
+    ev = np.linalg.eigvals(h.dense_one_electron())
+    logged = np.array([complex(a, b) for a, b in g["heplus_eigenvalues"]])
+    err = np.array([np.min(np.abs(ev - z)) for z in logged])
+    # The reference's block Lanczos ran to LANTOL = 1e-2 (CSCALE-WORK/Input.Inp.singlet:27),
+    # so a few members of degenerate clusters are printed unconverged (the log
+    # itself lists -0.219636044640111 and -0.219636021336936 for one 3-fold
+    # level).  Converged levels agree to rounding; every value to the log's spread.
+    assert np.sort(err)[6] < 1e-12 and np.sort(err)[11] < 1e-9
+    assert err[:3].max() < 1e-13
+    assert err.max() < 5e-6
+
+
+def test_hammult_matches_dense_kronecker_nn3():
+    h = Hamiltonian(1, 0.6, 0.3)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(h.d) + 1j * rng.standard_normal(h.d)
+    hd = h.dense()
+    np.testing.assert_allclose(h.hammult(x), hd @ x, rtol=0, atol=1e-12 * np.abs(hd @ x).max())
+    # exchange symmetry P H P = H and complex symmetry H^T = H
+    perm = exchange(np.arange(h.d), h.m)
+    np.testing.assert_allclose(hd[np.ix_(perm, perm)], hd, atol=1e-12)
+    np.testing.assert_allclose(hd.T, hd, atol=1e-12)
+
+
+def test_hammult_matches_sparse_kronecker_nn5():
+    h = Hamiltonian(2, 0.45, 0.35)
+    t = sp.csr_matrix(h.t.T)
+    i = sp.identity(h.nn, format="csr")
+    t3 = sp.kron(sp.kron(i, i), t) + sp.kron(sp.kron(i, t), i) + sp.kron(sp.kron(t, i), i)
+    im = sp.identity(h.m, format="csr")
+    a = sp.kron(im, t3) + sp.kron(t3, im) + sp.diags(h.potential2().reshape(-1, order="F"))
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(h.d) + 1j * rng.standard_normal(h.d)
+    ref = a @ x
+    np.testing.assert_allclose(h.hammult(x), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def _dense_singlet_eigs(h):
+    hd = h.dense()
+    w, v = np.linalg.eig(hd)
+    sym = np.array([np.linalg.norm(v[:, j] - exchange(v[:, j], h.m)) for j in range(v.shape[1])])
+    return w[sym < 1e-6]
+
+
+def test_gplmr_fixed_point_matches_dense_nn3():
+    h = Hamiltonian(1, 0.6, 0.3)
+    sigma = 0.5 - 0.1j
+    nev = 3
+    ws = _dense_singlet_eigs(h)
+    rng = np.random.default_rng(11)
+    x0 = rng.standard_normal((h.d, nev)) + 1j * rng.standard_normal((h.d, nev))
+    for j in range(nev):
+        x0[:, j] = singlet_project(x0[:, j], h.m)
+    hd = h.dense()
+    res = gplmr_eig(lambda b: hd @ b, x0, sigma, nev, ns=1, maxit=300, tol=1e-10)
+    assert res.converged
+    for e in res.e:
+        assert np.min(np.abs(ws - e)) < 1e-9
+    near = ws[np.argsort(np.abs(ws - sigma))[:nev]]
+    np.testing.assert_allclose(np.sort_complex(res.e), np.sort_complex(near), atol=1e-9)
+
+
+def test_orth_block_identities():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((500, 6)) + 1j * rng.standard_normal((500, 6))
+    q, t = orth_block(x.copy(), drop=False)
+    np.testing.assert_allclose(q.conj().T @ q, np.eye(6), atol=1e-12)
+    np.testing.assert_allclose(np.tril(t, -1), 0, atol=0)  # R^{-1} is upper triangular
+    np.testing.assert_allclose(x @ t, q, atol=1e-12)
+    # dependent columns are dropped on the fallback path
+    y = np.concatenate([x[:, :3], x[:, :1] * 2.0], axis=1)
+    q2, t2 = orth_block(y, drop=True)
+    assert q2.shape[1] == 3
+    np.testing.assert_allclose(q2.conj().T @ q2, np.eye(3), atol=1e-10)
