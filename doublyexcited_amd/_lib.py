@@ -28,7 +28,8 @@ DEX_FP32 = 1
 # every symbol include/dex.h declares
 EXPORTS = [
     "dex_params_default", "dex_ctx_create", "dex_ctx_destroy", "dex_ctx_info",
-    "dex_get_kematrix", "dex_get_tinv", "dex_matvec", "dex_matvec_device", "dex_dots",
+    "dex_get_kematrix", "dex_get_tinv", "dex_matvec", "dex_matvec_device", "dex_matvec_sym_device",
+    "dex_dots",
     "dex_dots_device", "dex_solve_opts_default", "dex_gplmr_eig", "dex_gplmr_eig_device",
     "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint",
     "dex_status_string", "dex_last_error", "dex_set_kernel_timing", "dex_get_kernel_timing",
@@ -59,7 +60,7 @@ class DexResult(C.Structure):
 class DexSolveOpts(C.Structure):
     _fields_ = [("strict_cholesky", C.c_int32), ("verbose", C.c_int32),
                 ("refresh_every", C.c_int32), ("fp64_restarts", C.c_int32),
-                ("fp32_tol", C.c_double)]
+                ("fp32_tol", C.c_double), ("symmetry", C.c_int32)]
 
 
 class DexError(RuntimeError):
@@ -89,6 +90,7 @@ def load() -> C.CDLL:
         "dex_get_tinv": (C.c_int, [P, dp]),
         "dex_matvec": (C.c_int, [P, dp, dp, C.c_int32, C.c_int64]),
         "dex_matvec_device": (C.c_int, [P, P, P, C.c_int32, C.c_int64, C.c_int32]),
+        "dex_matvec_sym_device": (C.c_int, [P, P, P, C.c_int32, C.c_int64, C.c_int32, C.c_int32]),
         "dex_dots": (C.c_int, [P, dp, dp, C.c_int64, C.c_int32, C.c_int32, dp]),
         "dex_dots_device": (C.c_int, [P, P, P, C.c_int64, C.c_int32, C.c_int32, dp]),
         "dex_solve_opts_default": (None, [C.POINTER(DexSolveOpts)]),

@@ -84,8 +84,16 @@ int ensure_single_staging(dex_ctx *c) {
   return 0;
 }
 
+// Second pass: the general electron-2 cube pass, or for X = sym X^T (sym = +-1)
+// the transpose-add pass.
+hipError_t second_pass(dex_ctx *c, bool single, const void *x, void *y, int sym) {
+  if (sym != 0)
+    return dex::launch_symadd(c->nn, single, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, sym, c->stream);
+  return dex::launch_pass2(c->nn, single, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, c->stream);
+}
+
 // One application of H in the requested arithmetic; x, y complex128 on device.
-int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single) {
+int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0) {
   if (int st = ensure_u(c)) return st;
   hipStream_t s = c->stream;
   if (single) {
@@ -95,7 +103,7 @@ int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single) {
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
     DEX_HIP(dex::launch_pass1(c->nn, true, c->d_xs, c->d_u, s));
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
-    DEX_HIP(dex::launch_pass2(c->nn, true, c->d_xs, c->d_u, c->d_ys, c->d_vn, c->d_tinv, c->coef, s));
+    DEX_HIP(second_pass(c, true, c->d_xs, c->d_ys, sym));
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
     DEX_HIP(dex::launch_to_double(c->d_ys, y, c->d, s));
     return 0;
@@ -104,7 +112,7 @@ int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single) {
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
     DEX_HIP(dex::launch_pass1(c->nn, false, x, c->d_u, s));
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
-    DEX_HIP(dex::launch_pass2(c->nn, false, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, s));
+    DEX_HIP(second_pass(c, false, x, y, sym));
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
     return 0;
   }
@@ -133,14 +141,22 @@ int gram(dex_ctx *c, const std::vector<ColDesc> &l, const std::vector<ColDesc> &
   const int kl = int(l.size()), kr = int(r.size());
   g.assign(size_t(kl) * kr, 0.0);
   if (kl == 0 || kr == 0) return 0;
-  std::vector<ColDesc> ds(l);
-  ds.insert(ds.end(), r.begin(), r.end());
-  if (int st = upload_descs(c, ds)) return st;
-  DEX_HIP(dex::launch_gram(c->d_desc, kl, kr, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
-                           c->d_gram_out, c->stream));
-  DEX_HIP(hipMemcpyAsync(g.data(), c->d_gram_out, g.size() * sizeof(Dbl2), hipMemcpyDeviceToHost,
-                         c->stream));
-  DEX_HIP(hipStreamSynchronize(c->stream));
+  // the kernel takes at most 64 x 128 outputs per launch
+  for (int i0 = 0; i0 < kl; i0 += 64)
+    for (int j0 = 0; j0 < kr; j0 += 128) {
+      const int bl = std::min(64, kl - i0), br = std::min(128, kr - j0);
+      std::vector<ColDesc> ds(l.begin() + i0, l.begin() + i0 + bl);
+      ds.insert(ds.end(), r.begin() + j0, r.begin() + j0 + br);
+      if (int st = upload_descs(c, ds)) return st;
+      DEX_HIP(dex::launch_gram(c->d_desc, bl, br, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
+                               c->d_gram_out, c->stream));
+      std::vector<cd> part(size_t(bl) * br);
+      DEX_HIP(hipMemcpyAsync(part.data(), c->d_gram_out, part.size() * sizeof(Dbl2),
+                             hipMemcpyDeviceToHost, c->stream));
+      DEX_HIP(hipStreamSynchronize(c->stream));
+      for (int j = 0; j < br; ++j)
+        for (int i = 0; i < bl; ++i) g[(i0 + i) + size_t(kl) * (j0 + j)] = part[i + size_t(bl) * j];
+    }
   return 0;
 }
 
@@ -366,6 +382,14 @@ int dex_get_kernel_timing(const dex_ctx *c, double *p1, double *p2) {
 
 int dex_matvec_device(dex_ctx *c, const void *in, void *out, int32_t howmany, int64_t checksize,
                       int32_t single) {
+  return dex_matvec_sym_device(c, in, out, howmany, checksize, single, 0);
+}
+
+int dex_matvec_sym_device(dex_ctx *c, const void *in, void *out, int32_t howmany, int64_t checksize,
+                          int32_t single, int32_t symmetry) {
+  if (symmetry < -1 || symmetry > 1) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
+  if (symmetry != 0 && !(c && (single ? c->fast32 : c->fast64)))
+    return fail(DEX_EUNSUPPORTED, "exchange-symmetric matvec needs the LDS kernels for this grid");
   if (!c || !in || !out) return fail(DEX_EINVAL, "null argument");
   if (checksize != c->d) return fail(DEX_ESIZE, "SIZE ERROR: checksize != M*MBLK");
   if (howmany < 0) return fail(DEX_EINVAL, "howmany < 0");
@@ -381,15 +405,16 @@ int dex_matvec_device(dex_ctx *c, const void *in, void *out, int32_t howmany, in
       if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], c->stream));
       DEX_HIP(dex::launch_pass1(c->nn, true, xi, c->d_u, c->stream));
       if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], c->stream));
-      DEX_HIP(dex::launch_pass2(c->nn, true, xi, c->d_u, yo, c->d_vn, c->d_tinv, c->coef, c->stream));
+      DEX_HIP(second_pass(c, true, xi, yo, symmetry));
       if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], c->stream));
     } else {
       const Dbl2 *xi = static_cast<const Dbl2 *>(in) + size_t(j) * c->d;
       Dbl2 *yo = static_cast<Dbl2 *>(out) + size_t(j) * c->d;
-      if (int st = apply_h(c, xi, yo, c->p.precision == DEX_FP32)) return st;
+      if (int st = apply_h(c, xi, yo, c->p.precision == DEX_FP32, symmetry)) return st;
     }
   }
-  if (c->timing && howmany > 0) {
+  const bool timed_fast = c->p.precision == DEX_FP32 ? c->fast32 : c->fast64;
+  if (c->timing && howmany > 0 && timed_fast) {
     DEX_HIP(hipEventSynchronize(c->ev[2]));
     float a = 0.f, b = 0.f;
     DEX_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -434,7 +459,7 @@ int dex_dots_device(dex_ctx *c, const void *bras, const void *kets, int64_t size
                     int32_t mm, double *out) {
   if (!c || !bras || !kets || !out) return fail(DEX_EINVAL, "null argument");
   if (size != c->d) return fail(DEX_ESIZE, "SIZE ERROR: size != M*MBLK");
-  if (nn < 1 || mm < 1 || nn + mm > 192) return fail(DEX_EINVAL, "nn, mm out of range");
+  if (nn < 1 || mm < 1) return fail(DEX_EINVAL, "nn, mm out of range");
   DEX_HIP(hipSetDevice(c->p.device));
   std::vector<ColDesc> l, r;
   for (int i = 0; i < nn; ++i) l.push_back(plain(static_cast<const Dbl2 *>(bras) + size_t(i) * size));
@@ -480,6 +505,7 @@ struct Solver {
   int nev, ns;
   bool strict;
   bool single;  // FP32 matvec
+  int sym;      // 0, or +-1 for exchange-(anti)symmetric blocks
   int fallbacks = 0;
   int64_t matvecs = 0;
   double t_matvec = 0, t_gram = 0, t_update = 0, t_small = 0;
@@ -488,7 +514,7 @@ struct Solver {
   int matblock(const Block &in, const Block &out) {
     const double t0 = now_s();
     for (int j = 0; j < in.size(); ++j) {
-      if (int st = apply_h(c, in.cols[j], out.cols[j], single)) return st;
+      if (int st = apply_h(c, in.cols[j], out.cols[j], single, sym)) return st;
       ++matvecs;
     }
     DEX_HIP(hipStreamSynchronize(c->stream));
@@ -652,7 +678,7 @@ struct Solver {
 int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int maxit, double tol,
               int inner, int outmodulus, const dex_solve_opts &o, dex_result *res, bool single,
               std::vector<Dbl2 *> &store) {
-  Solver S{c, nev, ns, o.strict_cholesky != 0, single};
+  Solver S{c, nev, ns, o.strict_cholesky != 0, single, o.symmetry};
   // every buffer in `store` is free at entry
   const size_t vbytes = size_t(c->d) * sizeof(Dbl2);
   // columns: X in xdev; the rest from `store` (pool of (5 + 2 ns) nev vectors)
@@ -664,6 +690,10 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
   if (int st = S.grow(W, nev)) return st;
   (void)vbytes;
   const double t_start = now_s();
+  if (o.symmetry != 0) {
+    // the exchange-symmetric matvec assumes X = sym X^T: project the start block
+    for (int j = 0; j < nev; ++j) DEX_HIP(dex::launch_sym_project(X.cols[j], c->m, o.symmetry, c->stream));
+  }
   // NORMALIZE(X) (gplmr.F90:65)
   {
     std::vector<double> aa, bb;
@@ -897,6 +927,9 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   if (int64_t(m) * mb != c->d) return fail(DEX_ESIZE, "SIZE ERROR: M*MB != NN^6");
   if (nev < 1 || ns < 1 || (3 + ns) * nev > 64) return fail(DEX_EINVAL, "need NEV >= 1, NS >= 1, (3+NS)*NEV <= 64");
   if (inner_iter != 0 && inner_iter != 1) return fail(DEX_EUNSUPPORTED, "GM_MAX_INNER_ITER must be 0 or 1");
+  if (opts && (opts->symmetry < -1 || opts->symmetry > 1)) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
+  if (opts && opts->symmetry != 0 && !c->fast64 && !(c->p.precision == DEX_FP32 && c->fast32))
+    return fail(DEX_EUNSUPPORTED, "exchange-symmetric matvec needs the LDS kernels for this grid");
   DEX_HIP(hipSetDevice(c->p.device));
   dex_solve_opts o;
   dex_solve_opts_default(&o);
@@ -1000,13 +1033,13 @@ extern "C" int dex_residuals_device(dex_ctx *c, const void *x_dev, const double 
 }
 
 extern "C" int dex_fill_gaussian_device(dex_ctx *c, void *x_dev, int32_t ncols, uint64_t seed,
-                                        uint32_t stream_id, int32_t singlet) {
+                                        uint32_t stream_id, int32_t symmetry) {
   if (!c || !x_dev) return fail(DEX_EINVAL, "null argument");
   DEX_HIP(hipSetDevice(c->p.device));
   for (int j = 0; j < ncols; ++j) {
     Dbl2 *xj = static_cast<Dbl2 *>(x_dev) + size_t(j) * c->d;
     DEX_HIP(dex::launch_fill_gaussian(xj, c->d, seed, uint32_t(j), stream_id, c->stream));
-    if (singlet) DEX_HIP(dex::launch_singlet_project(xj, c->m, c->stream));
+    if (symmetry != 0) DEX_HIP(dex::launch_sym_project(xj, c->m, symmetry > 0 ? 1 : -1, c->stream));
   }
   DEX_HIP(hipStreamSynchronize(c->stream));
   return DEX_OK;

@@ -84,8 +84,12 @@ hipError_t launch_coldots(const ColDesc *descs, int k, int64_t d, double *partia
 // (g0 + i g1)/sqrt(2) with g from Box-Muller on the counter (i, c, stream, 0).
 hipError_t launch_fill_gaussian(Dbl2 *x, int64_t n, uint64_t seed, uint32_t col, uint32_t stream_id,
                                 hipStream_t s);
-// Singlet projection X <- (X + X^T)/2 in the M x M view (in place).
-hipError_t launch_singlet_project(Dbl2 *x, int64_t m, hipStream_t s);
+// Exchange projection X <- (X + sign X^T)/2 in the M x M view (in place).
+hipError_t launch_sym_project(Dbl2 *x, int64_t m, int sign, hipStream_t s);
+// Second matvec pass for X = sign X^T: Y = s2 (U + sign U^T) + c1 Vr o (X + sign X^T)/2.
+hipError_t launch_symadd(int nn, bool single, const void *x, const void *u, void *y,
+                         const double *vn, const double *tinv, const MatvecCoef &coef, int sign,
+                         hipStream_t s);
 hipError_t launch_to_single(const Dbl2 *x, void *xs, int64_t n, hipStream_t s);
 hipError_t launch_to_double(const void *xs, Dbl2 *x, int64_t n, hipStream_t s);
 

@@ -236,6 +236,77 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass2(
   }
 }
 
+// ---- exchange-symmetric second pass ------------------------------------------
+// For X = sign X^T (sign = +1 singlet, -1 triplet) electron 2's kinetic term is
+// the transpose of electron 1's: (I x T3D) X = sign (T3D X)^T.  So after pass 1
+// (U = tau3D X) the result is Y = s2 (U + sign U^T) + c1 Vr o Xs with
+// Xs = (X + sign X^T) / 2, formed tile pair by tile pair: a workgroup owns the
+// 32 x 32 tile (ta, tb), ta <= tb, and its mirror (tb, ta); all global
+// accesses run along p (coalesced) and the transposes go through LDS.  Each
+// unordered pair {(p, q), (q, p)} is computed once and written with its sign,
+// so the output is exactly (anti)symmetric.
+constexpr int kSymT = 32;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_symadd(const typename Cplx<R>::type *__restrict__ x,
+                                                const typename Cplx<R>::type *__restrict__ u,
+                                                typename Cplx<R>::type *__restrict__ y, int64_t m,
+                                                int nn, const double *__restrict__ vn,
+                                                const double *__restrict__ tinv, MatvecCoef cf,
+                                                int sign) {
+  using C = typename Cplx<R>::type;
+  const int ta = blockIdx.x, tb = blockIdx.y;
+  if (ta > tb) return;
+  __shared__ C su[kSymT][kSymT + 1], sx[kSymT][kSymT + 1], sy[kSymT][kSymT + 1];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int64_t pa = int64_t(ta) * kSymT, qb = int64_t(tb) * kSymT;
+  // mirror tile: su[r][c] = U[q + m p] with q = qb + c, p = pa + r
+  for (int r = ty; r < kSymT; r += 8) {
+    const int64_t q = qb + tx, p = pa + r;
+    if (q < m && p < m) {
+      su[r][tx] = u[q + m * p];
+      sx[r][tx] = x[q + m * p];
+    }
+  }
+  __syncthreads();
+  const int L = 2 * nn + 1;
+  const R s2r = R(cf.s2_re), s2i = R(cf.s2_im), c1r = R(cf.c1_re), c1i = R(cf.c1_im);
+  const R sg = R(sign);
+  const bool diag = ta == tb;
+  for (int r = ty; r < kSymT; r += 8) {
+    const int64_t p = pa + tx, q = qb + r;  // main element (p, q) at p + m q
+    if (p < m && q < m) {
+      const C uv = u[p + m * q];
+      const C xv = x[p + m * q];
+      const C um = su[tx][r], xm = sx[tx][r];  // U[q + m p], X[q + m p]
+      const R sr = uv.x + sg * um.x, si = uv.y + sg * um.y;
+      const R xr = R(0.5) * (xv.x + sg * xm.x), xi = R(0.5) * (xv.y + sg * xm.y);
+      const int ip = int(p % nn), jp = int((p / nn) % nn), kp = int(p / (nn * nn));
+      const int iq = int(q % nn), jq = int((q / nn) % nn), kq = int(q / (nn * nn));
+      const int ti = (iq - ip + nn) + L * ((jq - jp + nn) + L * (kq - kp + nn));
+      const R vr = R(vn[p] + vn[q] + tinv[ti] * cf.inv_d3);
+      const R vxr = vr * xr, vxi = vr * xi;
+      C o;
+      o.x = s2r * sr - s2i * si + c1r * vxr - c1i * vxi;
+      o.y = s2r * si + s2i * sr + c1r * vxi + c1i * vxr;
+      sy[tx][r] = o;
+      if (!diag || tx <= r) y[p + m * q] = o;
+    }
+  }
+  __syncthreads();
+  // mirror write: y[q + m p] = sign * Y(p, q), q = qb + tx, p = pa + r
+  for (int r = ty; r < kSymT; r += 8) {
+    const int64_t q = qb + tx, p = pa + r;
+    if (q < m && p < m && (!diag || r < tx)) {
+      const C o = sy[r][tx];
+      C w;
+      w.x = sg * o.x;
+      w.y = sg * o.y;
+      y[q + m * p] = w;
+    }
+  }
+}
+
 // ---- generic FP64 path ------------------------------------------------------
 // One thread per line of the given axis (stride s): y (+)= tau * x along it.
 template <int NN>
@@ -410,6 +481,23 @@ hipError_t launch_pass2(int nn, bool single, const void *x, const void *u, void 
   }
 }
 
+hipError_t launch_symadd(int nn, bool single, const void *x, const void *u, void *y,
+                         const double *vn, const double *tinv, const MatvecCoef &coef, int sign,
+                         hipStream_t s) {
+  const int64_t m = int64_t(nn) * nn * nn;
+  const unsigned nt = unsigned((m + kSymT - 1) / kSymT);
+  if (single) {
+    hipLaunchKernelGGL(k_symadd<float>, dim3(nt, nt), dim3(kSymT, 8), 0, s,
+                       static_cast<const float2 *>(x), static_cast<const float2 *>(u),
+                       static_cast<float2 *>(y), m, nn, vn, tinv, coef, sign);
+  } else {
+    hipLaunchKernelGGL(k_symadd<double>, dim3(nt, nt), dim3(kSymT, 8), 0, s,
+                       static_cast<const double2 *>(x), static_cast<const double2 *>(u),
+                       static_cast<double2 *>(y), m, nn, vn, tinv, coef, sign);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *vn,
                                  const double *tinv, const MatvecCoef &coef, hipStream_t s) {
   const double2 *xx = reinterpret_cast<const double2 *>(x);
@@ -429,9 +517,9 @@ hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *v
 namespace {
 
 constexpr int kGramThreads = 256;
-constexpr int kGramRows = 32;        // rows per LDS tile
-constexpr int kGramMaxCols = 192;    // kl + kr
-constexpr int kGramTilesPerThread = 2;
+constexpr int kGramRows = 16;        // rows per LDS tile
+constexpr int kGramMaxL = 64;
+constexpr int kGramMaxR = 128;
 constexpr int kGramBlocks = 1024;
 
 __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
@@ -444,70 +532,61 @@ __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
   return o;
 }
 
-// Partial Gram over a contiguous row range per workgroup.  Output tiles of
-// 4 x 4 complex entries are dealt to threads; LDS holds kGramRows rows of all
-// kl + kr columns.
+// Partial Gram G = L^H R over a contiguous row range per workgroup.  The 256
+// threads form a 16 x 16 grid; thread (a, b) owns the outputs
+// (a + 16 m, b + 16 n), m < MA, n < NB, so each LDS row read is shared by many
+// FMAs and consecutive lanes read consecutive columns.  LDS holds kGramRows
+// rows of all kl + kr columns, row-major with one column of padding.
+template <int MA, int NB>
 __global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict__ descs, int kl,
                                                        int kr, int64_t d, int64_t rows_per_block,
                                                        double2 *__restrict__ partial) {
-  __shared__ double2 tile[kGramRows][kGramMaxCols + 1];
+  constexpr int LD = 16 * MA + 16 * NB + 1;
+  __shared__ double2 tile[kGramRows * LD];
   const int tid = threadIdx.x;
+  const int a = tid % 16, b = tid / 16;
   const int ncol = kl + kr;
-  const int tl = (kl + 3) / 4, trr = (kr + 3) / 4;
-  const int ntiles = tl * trr;
-  double2 acc[kGramTilesPerThread][4][4];
+  double accr[MA][NB], acci[MA][NB];
 #pragma unroll
-  for (int t = 0; t < kGramTilesPerThread; ++t)
+  for (int m = 0; m < MA; ++m)
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[t][a][b] = make_double2(0.0, 0.0);
+    for (int n = 0; n < NB; ++n) accr[m][n] = acci[m][n] = 0.0;
   const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
   for (int64_t rb = r0; rb < r1; rb += kGramRows) {
     const int nr = int(r1 - rb < kGramRows ? r1 - rb : kGramRows);
     for (int e = tid; e < kGramRows * ncol; e += kGramThreads) {
       const int r = e % kGramRows, c = e / kGramRows;
-      tile[r][c] = r < nr ? col_value(descs[c], rb + r) : make_double2(0.0, 0.0);
+      // L columns at [0, 16 MA), R columns at [16 MA, 16 MA + 16 NB)
+      const int slot = c < kl ? c : 16 * MA + (c - kl);
+      tile[r * LD + slot] = r < nr ? col_value(descs[c], rb + r) : make_double2(0.0, 0.0);
     }
     __syncthreads();
+#pragma unroll 4
+    for (int r = 0; r < kGramRows; ++r) {
+      double2 lv[MA], rv[NB];
 #pragma unroll
-    for (int t = 0; t < kGramTilesPerThread; ++t) {
-      const int tix = tid + t * kGramThreads;
-      if (tix < ntiles) {
-        const int i0 = (tix % tl) * 4, j0 = (tix / tl) * 4;
-        for (int r = 0; r < kGramRows; ++r) {
-          double2 lv[4], rv[4];
+      for (int m = 0; m < MA; ++m) lv[m] = tile[r * LD + a + 16 * m];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) lv[a] = tile[r][i0 + a < kl ? i0 + a : 0];
+      for (int n = 0; n < NB; ++n) rv[n] = tile[r * LD + 16 * MA + b + 16 * n];
 #pragma unroll
-          for (int b = 0; b < 4; ++b) rv[b] = tile[r][kl + (j0 + b < kr ? j0 + b : 0)];
+      for (int m = 0; m < MA; ++m)
 #pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-              // conj(l) * r
-              acc[t][a][b].x += lv[a].x * rv[b].x + lv[a].y * rv[b].y;
-              acc[t][a][b].y += lv[a].x * rv[b].y - lv[a].y * rv[b].x;
-            }
+        for (int n = 0; n < NB; ++n) {
+          accr[m][n] += lv[m].x * rv[n].x + lv[m].y * rv[n].y;
+          acci[m][n] += lv[m].x * rv[n].y - lv[m].y * rv[n].x;
         }
-      }
     }
     __syncthreads();
   }
   double2 *out = partial + size_t(blockIdx.x) * kl * kr;
 #pragma unroll
-  for (int t = 0; t < kGramTilesPerThread; ++t) {
-    const int tix = tid + t * kGramThreads;
-    if (tix < ntiles) {
-      const int i0 = (tix % tl) * 4, j0 = (tix / tl) * 4;
+  for (int m = 0; m < MA; ++m)
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (i0 + a < kl && j0 + b < kr) out[(i0 + a) + size_t(kl) * (j0 + b)] = acc[t][a][b];
+    for (int n = 0; n < NB; ++n) {
+      const int i = a + 16 * m, j = b + 16 * n;
+      if (i < kl && j < kr) out[i + size_t(kl) * j] = make_double2(accr[m][n], acci[m][n]);
     }
-  }
 }
 
 __global__ void k_reduce_partials(const double2 *__restrict__ partial, int nparts, int nelem,
@@ -527,39 +606,59 @@ constexpr int kCombRows = 64;
 constexpr int kCombThreads = 256;
 constexpr int kCombMaxIn = 96;
 constexpr int kCombMaxOut = 48;
+constexpr int kCombJ = 8;  // outputs per thread
 
+// out_j = sum_i in_i C(i, j) (+ beta_j out_j).  A workgroup takes 64 rows;
+// thread t owns row t % 64 and the 8 outputs starting at 8 (t / 64) (then
+// +32, ...), so C(i, j) is uniform across a wave.  The input tile is stored
+// column by column in LDS (conflict-free reads by consecutive rows).
 __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restrict__ descs, int nin,
                                                           int nout, const double2 *__restrict__ cm,
                                                           const double2 *__restrict__ beta, int64_t d) {
-  __shared__ double2 tin[kCombRows][kCombMaxIn + 1];
+  __shared__ double2 tin[kCombMaxIn * kCombRows];
   const int tid = threadIdx.x;
+  const int row = tid % kCombRows;
+  const int wj = __builtin_amdgcn_readfirstlane(tid / kCombRows);
   const int64_t ntile = (d + kCombRows - 1) / kCombRows;
   for (int64_t tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
     const int64_t rb = tb * kCombRows;
     const int nr = int(d - rb < kCombRows ? d - rb : kCombRows);
     for (int e = tid; e < kCombRows * nin; e += kCombThreads) {
       const int r = e % kCombRows, c = e / kCombRows;
-      if (r < nr) tin[r][c] = col_value(descs[c], rb + r);
+      if (r < nr) tin[c * kCombRows + r] = col_value(descs[c], rb + r);
     }
     __syncthreads();
-    for (int e = tid; e < kCombRows * nout; e += kCombThreads) {
-      const int r = e % kCombRows, j = e / kCombRows;
-      if (r < nr) {
-        double sx = 0.0, sy = 0.0;
+    if (row < nr) {
+      for (int j0 = wj * kCombJ; j0 < nout; j0 += (kCombThreads / kCombRows) * kCombJ) {
+        double sx[kCombJ], sy[kCombJ];
+#pragma unroll
+        for (int q = 0; q < kCombJ; ++q) sx[q] = sy[q] = 0.0;
         for (int i = 0; i < nin; ++i) {
-          const double2 a = tin[r][i];
-          const double2 c = cm[i + size_t(nin) * j];
-          sx += a.x * c.x - a.y * c.y;
-          sy += a.x * c.y + a.y * c.x;
+          const double2 a = tin[i * kCombRows + row];
+#pragma unroll
+          for (int q = 0; q < kCombJ; ++q) {
+            if (j0 + q < nout) {
+              const double2 c = cm[i + size_t(nin) * (j0 + q)];
+              sx[q] += a.x * c.x - a.y * c.y;
+              sy[q] += a.x * c.y + a.y * c.x;
+            }
+          }
         }
-        double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + j].a)) + rb + r;
-        if (beta != nullptr) {
-          const double2 bj = beta[j];
-          const double2 ov = *o;
-          sx += bj.x * ov.x - bj.y * ov.y;
-          sy += bj.x * ov.y + bj.y * ov.x;
+#pragma unroll
+        for (int q = 0; q < kCombJ; ++q) {
+          const int j = j0 + q;
+          if (j < nout) {
+            double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + j].a)) + rb + row;
+            double ox = sx[q], oy = sy[q];
+            if (beta != nullptr) {
+              const double2 bj = beta[j];
+              const double2 ov = *o;
+              ox += bj.x * ov.x - bj.y * ov.y;
+              oy += bj.x * ov.y + bj.y * ov.x;
+            }
+            *o = make_double2(ox, oy);
+          }
         }
-        *o = make_double2(sx, sy);
       }
     }
     __syncthreads();
@@ -655,8 +754,8 @@ __global__ void k_fill_gaussian(double2 *__restrict__ x, int64_t n, uint64_t see
   x[i] = make_double2(rad * cs * s, rad * sn * s);
 }
 
-// X <- (X + X^T)/2 on the M x M matrix, tiles of 32 x 32 above the diagonal.
-__global__ void k_singlet(double2 *__restrict__ x, int64_t m) {
+// X <- (X + sign X^T)/2 on the M x M matrix, tiles of 32 x 32 above the diagonal.
+__global__ void k_singlet(double2 *__restrict__ x, int64_t m, double sign) {
   __shared__ double2 ta[32][33], tb[32][33];
   const int64_t bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
@@ -671,11 +770,11 @@ __global__ void k_singlet(double2 *__restrict__ x, int64_t m) {
   for (int r = ty; r < 32; r += 8) {
     const int64_t i = bi * 32 + tx, j = bj * 32 + r;
     const double2 a = ta[r][tx], b = tb[tx][r];  // b = x[j + m i]
-    const double2 s = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+    const double2 s = make_double2(0.5 * (a.x + sign * b.x), 0.5 * (a.y + sign * b.y));
     if (i < m && j < m) x[i + m * j] = s;
     const int64_t i2 = bj * 32 + tx, j2 = bi * 32 + r;
     const double2 a2 = tb[r][tx], b2 = ta[tx][r];
-    const double2 s2 = make_double2(0.5 * (a2.x + b2.x), 0.5 * (a2.y + b2.y));
+    const double2 s2 = make_double2(0.5 * (a2.x + sign * b2.x), 0.5 * (a2.y + sign * b2.y));
     if (i2 < m && j2 < m) x[i2 + m * j2] = s2;
   }
 }
@@ -696,14 +795,43 @@ int gram_blocks() { return kGramBlocks; }
 
 size_t gram_partial_elems(int kl, int kr) { return size_t(kGramBlocks) * kl * kr; }
 
+namespace {
+template <int MA, int NB>
+void gram_launch(const ColDesc *descs, int kl, int kr, int64_t d, int64_t rpb, double2 *partial,
+                 hipStream_t s) {
+  hipLaunchKernelGGL((k_gram<MA, NB>), dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr, d,
+                     rpb, partial);
+}
+
+template <int MA>
+void gram_launch_nb(int nb, const ColDesc *descs, int kl, int kr, int64_t d, int64_t rpb,
+                    double2 *partial, hipStream_t s) {
+  switch (nb) {
+    case 1: gram_launch<MA, 1>(descs, kl, kr, d, rpb, partial, s); break;
+    case 2: gram_launch<MA, 2>(descs, kl, kr, d, rpb, partial, s); break;
+    case 3: gram_launch<MA, 3>(descs, kl, kr, d, rpb, partial, s); break;
+    case 4: gram_launch<MA, 4>(descs, kl, kr, d, rpb, partial, s); break;
+    case 5: gram_launch<MA, 5>(descs, kl, kr, d, rpb, partial, s); break;
+    case 6: gram_launch<MA, 6>(descs, kl, kr, d, rpb, partial, s); break;
+    case 7: gram_launch<MA, 7>(descs, kl, kr, d, rpb, partial, s); break;
+    default: gram_launch<MA, 8>(descs, kl, kr, d, rpb, partial, s); break;
+  }
+}
+}  // namespace
+
 hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int64_t d, Dbl2 *partial, Dbl2 *out,
                        hipStream_t s) {
-  if (kl < 1 || kr < 1 || kl + kr > kGramMaxCols) return hipErrorInvalidValue;
-  if (((kl + 3) / 4) * ((kr + 3) / 4) > kGramThreads * kGramTilesPerThread) return hipErrorInvalidValue;
+  if (kl < 1 || kr < 1 || kl > kGramMaxL || kr > kGramMaxR) return hipErrorInvalidValue;
   int64_t rpb = (d + kGramBlocks - 1) / kGramBlocks;
   rpb = (rpb + kGramRows - 1) / kGramRows * kGramRows;
-  hipLaunchKernelGGL(k_gram, dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr, d, rpb,
-                     reinterpret_cast<double2 *>(partial));
+  const int ma = (kl + 15) / 16, nb = (kr + 15) / 16;
+  double2 *pp = reinterpret_cast<double2 *>(partial);
+  switch (ma) {
+    case 1: gram_launch_nb<1>(nb, descs, kl, kr, d, rpb, pp, s); break;
+    case 2: gram_launch_nb<2>(nb, descs, kl, kr, d, rpb, pp, s); break;
+    case 3: gram_launch_nb<3>(nb, descs, kl, kr, d, rpb, pp, s); break;
+    default: gram_launch_nb<4>(nb, descs, kl, kr, d, rpb, pp, s); break;
+  }
   const int nelem = kl * kr;
   hipLaunchKernelGGL(k_reduce_partials, dim3((nelem + 255) / 256), dim3(256), 0, s,
                      reinterpret_cast<const double2 *>(partial), kGramBlocks, nelem,
@@ -741,9 +869,10 @@ hipError_t launch_fill_gaussian(Dbl2 *x, int64_t n, uint64_t seed, uint32_t col,
   return hipGetLastError();
 }
 
-hipError_t launch_singlet_project(Dbl2 *x, int64_t m, hipStream_t s) {
+hipError_t launch_sym_project(Dbl2 *x, int64_t m, int sign, hipStream_t s) {
   const unsigned nb = unsigned((m + 31) / 32);
-  hipLaunchKernelGGL(k_singlet, dim3(nb, nb), dim3(32, 8), 0, s, reinterpret_cast<double2 *>(x), m);
+  hipLaunchKernelGGL(k_singlet, dim3(nb, nb), dim3(32, 8), 0, s, reinterpret_cast<double2 *>(x), m,
+                     double(sign));
   return hipGetLastError();
 }
 

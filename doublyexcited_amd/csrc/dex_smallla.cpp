@@ -163,82 +163,117 @@ int zgeev_right(int n, const cd *a_in, int lda, std::vector<cd> &lam, std::vecto
     }
     for (int i = k + 2; i < n; ++i) H(i, k) = 0.0;
   }
-  // Shifted QR on the Hessenberg matrix
-  const double eps = 2.220446049250313e-16;
-  int hi = n - 1;
-  int iter = 0, total = 0;
-  while (hi > 0) {
-    int l = hi;
-    while (l > 0) {
-      const double s = std::abs(H(l - 1, l - 1)) + std::abs(H(l, l));
-      const double tol = eps * (s > 0 ? s : anorm);
-      if (std::abs(H(l, l - 1)) <= tol) {
-        H(l, l - 1) = 0.0;
+  // Shifted QR on the Hessenberg matrix.  Deflation uses the Ahues-Tisseur
+  // criterion (SIAM J. Matrix Anal. Appl. 1997; as in LAPACK's ZLAHQR), which
+  // also deflates multiple eigenvalues promptly; every 10th step without
+  // deflation takes an exceptional shift.
+  const double ulp = 2.220446049250313e-16;
+  const double smlnum = 1e-300 * (double(n) / ulp);
+  auto cabs1 = [](const cd &z) { return std::fabs(z.real()) + std::fabs(z.imag()); };
+  const int itmax = 30 * std::max(10, n);
+  const int kexsh = 10;
+  int kdefl = 0;
+  std::vector<double> cs(n);
+  std::vector<cd> sn(n);
+  int i = n - 1;
+  while (i >= 0) {
+    int l = 0;
+    bool converged = false;
+    for (int its = 0; its <= itmax; ++its) {
+      int k;
+      for (k = i; k > l; --k) {
+        if (cabs1(H(k, k - 1)) <= smlnum) break;
+        double tst = cabs1(H(k - 1, k - 1)) + cabs1(H(k, k));
+        if (tst == 0.0) {
+          if (k - 2 >= l) tst += std::fabs(H(k - 1, k - 2).real());
+          if (k + 1 <= i) tst += std::fabs(H(k + 1, k).real());
+        }
+        if (std::fabs(H(k, k - 1).real()) <= ulp * tst) {
+          const double ab = std::max(cabs1(H(k, k - 1)), cabs1(H(k - 1, k)));
+          const double ba = std::min(cabs1(H(k, k - 1)), cabs1(H(k - 1, k)));
+          const double aa = std::max(cabs1(H(k, k)), cabs1(H(k - 1, k - 1) - H(k, k)));
+          const double bb = std::min(cabs1(H(k, k)), cabs1(H(k - 1, k - 1) - H(k, k)));
+          const double sden = aa + ab;
+          if (ba * (ab / sden) <= std::max(smlnum, ulp * (bb * (aa / sden)))) break;
+        }
+      }
+      l = k;
+      if (l > 0) H(l, l - 1) = 0.0;
+      if (l >= i) {
+        converged = true;
         break;
       }
-      --l;
-    }
-    if (l == hi) {
-      --hi;
-      iter = 0;
-      continue;
-    }
-    if (++total > 100 * n) return 1;
-    ++iter;
-    cd mu;
-    if (iter % 11 == 10) {
-      mu = H(hi, hi) + cd(std::abs(std::real(H(hi, hi - 1))) + std::abs(H(hi - 1, hi - 2 >= 0 ? hi - 2 : 0)), 0.0);
-    } else {
-      const cd a = H(hi - 1, hi - 1), b = H(hi - 1, hi), c = H(hi, hi - 1), d = H(hi, hi);
-      const cd tr = a + d, det = a * d - b * c;
-      const cd disc = std::sqrt(tr * tr * 0.25 - det);
-      const cd e1 = tr * 0.5 + disc, e2 = tr * 0.5 - disc;
-      mu = std::abs(e1 - d) < std::abs(e2 - d) ? e1 : e2;
-    }
-    // QR step on rows/cols l..hi via Givens rotations, applied to the full matrix
-    std::vector<double> cs(hi - l);
-    std::vector<cd> sn(hi - l);
-    for (int k = l; k <= hi; ++k) H(k, k) -= mu;
-    for (int k = l; k < hi; ++k) {
-      const cd x = H(k, k), y = H(k + 1, k);
-      const double r = std::sqrt(std::norm(x) + std::norm(y));
-      double c;
-      cd s;
-      if (r == 0.0) {
-        c = 1.0;
-        s = 0.0;
+      ++kdefl;
+      cd mu;
+      if (kdefl % (2 * kexsh) == 0) {
+        mu = H(i, i) + 0.75 * std::fabs(H(i, i - 1).real());
+      } else if (kdefl % kexsh == 0) {
+        mu = H(l, l) + 0.75 * std::fabs(H(l + 1, l).real());
       } else {
-        c = std::abs(x) / r;
-        const cd xph = std::abs(x) > 0 ? x / std::abs(x) : cd(1.0);
-        s = xph * std::conj(y) / r;
+        // Eigenvalue of the trailing 2 x 2 block [[a, b], [c, d]] nearer d,
+        // written as d - u^2 / (x + y) with x = (a - d)/2, u^2 = b c and
+        // y = sqrt(x^2 + u^2) (no cancellation between nearly equal terms).
+        mu = H(i, i);
+        const cd u = std::sqrt(H(i - 1, i)) * std::sqrt(H(i, i - 1));
+        double sc = cabs1(u);
+        if (sc != 0.0) {
+          const cd x = 0.5 * (H(i - 1, i - 1) - mu);
+          const double sx = cabs1(x);
+          sc = std::max(sc, sx);
+          const cd xs = x / sc, us = u / sc;
+          cd y = sc * std::sqrt(xs * xs + us * us);
+          if (sx > 0.0) {
+            const cd xn = x / sx;
+            if (xn.real() * y.real() + xn.imag() * y.imag() < 0.0) y = -y;
+          }
+          mu -= u * (u / (x + y));
+        }
       }
-      cs[k - l] = c;
-      sn[k - l] = s;
-      // rows k, k+1: [c s; -conj(s) c]
-      for (int j = k; j < n; ++j) {
-        const cd hk = H(k, j), hk1 = H(k + 1, j);
-        H(k, j) = c * hk + s * hk1;
-        H(k + 1, j) = -std::conj(s) * hk + c * hk1;
+      // explicit shifted QR step on rows/cols l..i, applied to the full matrix
+      for (int kk = l; kk <= i; ++kk) H(kk, kk) -= mu;
+      for (int kk = l; kk < i; ++kk) {
+        const cd x = H(kk, kk), y = H(kk + 1, kk);
+        const double r = std::sqrt(std::norm(x) + std::norm(y));
+        double c;
+        cd sv;
+        if (r == 0.0) {
+          c = 1.0;
+          sv = 0.0;
+        } else {
+          c = std::abs(x) / r;
+          const cd xph = std::abs(x) > 0 ? x / std::abs(x) : cd(1.0);
+          sv = xph * std::conj(y) / r;
+        }
+        cs[kk] = c;
+        sn[kk] = sv;
+        for (int j = kk; j < n; ++j) {
+          const cd hk = H(kk, j), hk1 = H(kk + 1, j);
+          H(kk, j) = c * hk + sv * hk1;
+          H(kk + 1, j) = -std::conj(sv) * hk + c * hk1;
+        }
       }
+      for (int kk = l; kk < i; ++kk) {
+        const double c = cs[kk];
+        const cd sv = sn[kk];
+        const int rmax = std::min(kk + 2, i);
+        for (int r = 0; r <= rmax; ++r) {
+          const cd hk = H(r, kk), hk1 = H(r, kk + 1);
+          H(r, kk) = c * hk + std::conj(sv) * hk1;
+          H(r, kk + 1) = -sv * hk + c * hk1;
+        }
+        for (int r = 0; r < n; ++r) {
+          const cd zk = Z(r, kk), zk1 = Z(r, kk + 1);
+          Z(r, kk) = c * zk + std::conj(sv) * zk1;
+          Z(r, kk + 1) = -sv * zk + c * zk1;
+        }
+      }
+      for (int kk = l; kk <= i; ++kk) H(kk, kk) += mu;
     }
-    for (int k = l; k < hi; ++k) {
-      const double c = cs[k - l];
-      const cd s = sn[k - l];
-      // columns k, k+1 times G^H
-      const int rmax = std::min(k + 2, hi);
-      for (int i = 0; i <= rmax; ++i) {
-        const cd hk = H(i, k), hk1 = H(i, k + 1);
-        H(i, k) = c * hk + std::conj(s) * hk1;
-        H(i, k + 1) = -s * hk + c * hk1;
-      }
-      for (int i = 0; i < n; ++i) {
-        const cd zk = Z(i, k), zk1 = Z(i, k + 1);
-        Z(i, k) = c * zk + std::conj(s) * zk1;
-        Z(i, k + 1) = -s * zk + c * zk1;
-      }
-    }
-    for (int k = l; k <= hi; ++k) H(k, k) += mu;
+    if (!converged) return 1;
+    kdefl = 0;
+    i = l - 1;
   }
+  const double eps = ulp;
   lam.resize(n);
   for (int i = 0; i < n; ++i) lam[i] = H(i, i);
   // eigenvectors of the triangular T, then back-transform
@@ -315,43 +350,50 @@ static int zgesv(int n, std::vector<cd> a, int m, cd *b, int ldb) {
 int harmonic_pencil(int k, const cd *kp, const cd *mp, int ld, int nev, std::vector<cd> &coords,
                     std::vector<cd> &mu) {
   if (nev > k) return DEX_EGEVP;
-  std::vector<cd> r(size_t(k) * k), c(size_t(k) * k);
-  for (int j = 0; j < k; ++j)
-    for (int i = 0; i < k; ++i) r[i + size_t(k) * j] = kp[i + size_t(ld) * j];
   std::vector<cd> lam, vec;
   std::vector<double> key(k);
-  bool chol = zpotrf_upper(k, r.data(), k) == 0;
-  if (chol) {
-    std::vector<cd> rinv = r;
-    ztrtri_upper(k, rinv.data(), k);
-    // C = R^{-H} M R^{-1}
-    std::vector<cd> t(size_t(k) * k, 0.0);
+  bool ok = false;
+  // Route 1: K = R^H R, C = R^{-H} M R^{-1}, nu = 1/mu, y = R^{-1} z.
+  {
+    std::vector<cd> r(size_t(k) * k), c(size_t(k) * k);
     for (int j = 0; j < k; ++j)
-      for (int i = 0; i < k; ++i) {
-        cd s = 0.0;
-        for (int l = 0; l <= j; ++l) s += mp[i + size_t(ld) * l] * rinv[l + size_t(k) * j];
-        t[i + size_t(k) * j] = s;
+      for (int i = 0; i < k; ++i) r[i + size_t(k) * j] = kp[i + size_t(ld) * j];
+    if (zpotrf_upper(k, r.data(), k) == 0) {
+      std::vector<cd> rinv = r;
+      ztrtri_upper(k, rinv.data(), k);
+      std::vector<cd> t(size_t(k) * k, 0.0);
+      for (int j = 0; j < k; ++j)
+        for (int i = 0; i < k; ++i) {
+          cd s = 0.0;
+          for (int l = 0; l <= j; ++l) s += mp[i + size_t(ld) * l] * rinv[l + size_t(k) * j];
+          t[i + size_t(k) * j] = s;
+        }
+      for (int j = 0; j < k; ++j)
+        for (int i = 0; i < k; ++i) {
+          cd s = 0.0;
+          for (int l = 0; l <= i; ++l) s += std::conj(rinv[l + size_t(k) * i]) * t[l + size_t(k) * j];
+          c[i + size_t(k) * j] = s;
+        }
+      if (zgeev_right(k, c.data(), k, lam, vec) == 0) {
+        std::vector<cd> y(size_t(k) * k);
+        for (int j = 0; j < k; ++j)
+          for (int i = 0; i < k; ++i) {
+            cd s = 0.0;
+            for (int l = i; l < k; ++l) s += rinv[i + size_t(k) * l] * vec[l + size_t(k) * j];
+            y[i + size_t(k) * j] = s;
+          }
+        vec.swap(y);
+        for (int i = 0; i < k; ++i) {
+          key[i] = std::abs(lam[i]) > 0 ? 1.0 / std::abs(lam[i]) : HUGE_VAL;
+          lam[i] = std::abs(lam[i]) > 0 ? 1.0 / lam[i] : cd(HUGE_VAL);
+        }
+        ok = true;
       }
-    for (int j = 0; j < k; ++j)
-      for (int i = 0; i < k; ++i) {
-        cd s = 0.0;
-        for (int l = 0; l <= i; ++l) s += std::conj(rinv[l + size_t(k) * i]) * t[l + size_t(k) * j];
-        c[i + size_t(k) * j] = s;
-      }
-    if (zgeev_right(k, c.data(), k, lam, vec) != 0) return DEX_EGEVP;
-    for (int i = 0; i < k; ++i) key[i] = std::abs(lam[i]) > 0 ? 1.0 / std::abs(lam[i]) : HUGE_VAL;
-    // y = R^{-1} z
-    std::vector<cd> y(size_t(k) * k);
-    for (int j = 0; j < k; ++j)
-      for (int i = 0; i < k; ++i) {
-        cd s = 0.0;
-        for (int l = i; l < k; ++l) s += rinv[i + size_t(k) * l] * vec[l + size_t(k) * j];
-        y[i + size_t(k) * j] = s;
-      }
-    vec.swap(y);
-    for (int i = 0; i < k; ++i) lam[i] = std::abs(lam[i]) > 0 ? 1.0 / lam[i] : cd(HUGE_VAL);
-  } else {
-    std::vector<cd> m(size_t(k) * k);
+    }
+  }
+  // Route 2: M^{-1} K by LU with partial pivoting.
+  if (!ok) {
+    std::vector<cd> m(size_t(k) * k), c(size_t(k) * k);
     for (int j = 0; j < k; ++j)
       for (int i = 0; i < k; ++i) {
         m[i + size_t(k) * j] = mp[i + size_t(ld) * j];

@@ -131,10 +131,13 @@ class HeliumGrid:
         return out
 
     # device variants (torch tensors as memory)
-    def matvec_device(self, x, y, howmany: int | None = None, single: bool = False) -> None:
+    def matvec_device(self, x, y, howmany: int | None = None, single: bool = False,
+                      symmetry: int = 0) -> None:
+        """HAMMULT on device tensors; symmetry = +1/-1 uses the exchange shortcut."""
         howmany = x.numel() // self.d if howmany is None else howmany
-        check(self._lib.dex_matvec_device(self._ctx, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
-                                          howmany, self.d, int(single)))
+        check(self._lib.dex_matvec_sym_device(self._ctx, C.c_void_p(x.data_ptr()),
+                                              C.c_void_p(y.data_ptr()), howmany, self.d, int(single),
+                                              int(symmetry)))
 
     def dots_device(self, bras, kets, nn: int, mm: int) -> np.ndarray:
         out = np.zeros((nn, mm), dtype=np.complex128, order="F")
@@ -142,9 +145,9 @@ class HeliumGrid:
                                         self.d, nn, mm, out.ctypes.data_as(_dp)))
         return out
 
-    def fill_gaussian_device(self, x, ncols: int, seed: int, stream_id: int = 0, singlet: bool = True):
+    def fill_gaussian_device(self, x, ncols: int, seed: int, stream_id: int = 0, symmetry: int = 1):
         check(self._lib.dex_fill_gaussian_device(self._ctx, C.c_void_p(x.data_ptr()), ncols, seed,
-                                                 stream_id, int(singlet)))
+                                                 stream_id, int(symmetry)))
 
     def sync(self):
         check(self._lib.dex_sync(self._ctx))
@@ -170,9 +173,11 @@ class HeliumGrid:
 
     # -- eigensolver --------------------------------------------------------------------
     @staticmethod
-    def _opts(strict: bool, verbose: bool, refresh_every: int, fp64_restarts: int, fp32_tol: float):
+    def _opts(strict: bool, verbose: bool, refresh_every: int, fp64_restarts: int, fp32_tol: float,
+              symmetry: int = 0):
         o = _lib.DexSolveOpts()
         _lib.load().dex_solve_opts_default(C.byref(o))
+        o.symmetry = int(symmetry)
         o.strict_cholesky = int(strict)
         o.verbose = int(verbose)
         o.refresh_every = refresh_every
@@ -190,7 +195,8 @@ class HeliumGrid:
     def gplmr_eig(self, x: np.ndarray, sigma: complex, nev: int, ns: int = 1, maxit: int = 100,
                   tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5, strict: bool = False,
                   verbose: bool = False, refresh_every: int = 0, fp64_restarts: int = 2,
-                  fp32_tol: float = 1e-5, m: int | None = None, mb: int | None = None) -> SolveResult:
+                  fp32_tol: float = 1e-5, m: int | None = None, mb: int | None = None,
+                  symmetry: int = 0) -> SolveResult:
         """GPLMR_EIG on a host block ``x`` (D*NEV complex128, vector-major), updated in place."""
         if x.dtype != np.complex128 or not (x.flags.f_contiguous or x.flags.c_contiguous):
             raise TypeError("x must be a contiguous complex128 array")
@@ -200,7 +206,7 @@ class HeliumGrid:
             raise DexError(_lib.DEX_ESIZE, "x.size != M*MB*NEV")
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
-        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol)
+        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry)
         st = self._lib.dex_gplmr_eig(self._ctx, _ptr(x), e.view(np.float64).ctypes.data_as(_dp), m, mb,
                                      float(np.real(sigma)), float(np.imag(sigma)), nev, ns, maxit, tol,
                                      inner_iter, outmodulus, C.byref(o), C.byref(r))
@@ -210,11 +216,12 @@ class HeliumGrid:
     def gplmr_eig_device(self, x, sigma: complex, nev: int, ns: int = 1, maxit: int = 100,
                          tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5,
                          strict: bool = False, verbose: bool = False, refresh_every: int = 0,
-                         fp64_restarts: int = 2, fp32_tol: float = 1e-5) -> SolveResult:
+                         fp64_restarts: int = 2, fp32_tol: float = 1e-5,
+                         symmetry: int = 0) -> SolveResult:
         """GPLMR_EIG with the block resident on the device (torch complex128 tensor)."""
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
-        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol)
+        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry)
         st = self._lib.dex_gplmr_eig_device(self._ctx, C.c_void_p(x.data_ptr()),
                                             e.view(np.float64).ctypes.data_as(_dp), self.m, self.m,
                                             float(np.real(sigma)), float(np.imag(sigma)), nev, ns,

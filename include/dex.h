@@ -91,6 +91,16 @@ int dex_matvec(dex_ctx *ctx, const double *in, double *out, int32_t howmany, int
 int dex_matvec_device(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
                       int64_t checksize, int32_t single);
 
+/* HAMMULT for exchange-(anti)symmetric vectors, X(P,Q) = symmetry * X(Q,P)
+ * (symmetry = +1 singlet, -1 triplet; 0 is dex_matvec_device).  Electron 2's
+ * kinetic term is then the transpose of electron 1's, so the second pass is a
+ * coalesced transpose-add; the result is Y = H Xs with Xs = (X + symmetry X^T)/2,
+ * exactly (anti)symmetric.  This is how the reference's start vectors enter
+ * (HELIUM_ORBITAL with SIGN = +-1, hammod.F90:1059-1063); HAMMULT itself has
+ * no such mode.  Needs the LDS kernels for the grid (dex_info.fast_path). */
+int dex_matvec_sym_device(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
+                          int64_t checksize, int32_t single, int32_t symmetry);
+
 /* MYDOTSUB(BRAS,KETS,SIZE,NN,MM,OUTDOTS), doublyexcited.F90:47-59:
  * OUT(NN,MM) = BRAS^H KETS, column-major.  Host or device pointers. */
 int dex_dots(dex_ctx *ctx, const double *bras, const double *kets, int64_t size,
@@ -119,6 +129,8 @@ typedef struct dex_solve_opts {
   int32_t refresh_every;   /* >0: recompute AX = H X every this many iterations (0 = never, as the reference) */
   int32_t fp64_restarts;   /* FP32 contexts: at most this many FP64 restarts after the FP32 phase */
   double fp32_tol;         /* FP32 contexts: residual target of the FP32-matvec phase */
+  int32_t symmetry;        /* 0: general HAMMULT; +1 / -1: project the start block to
+                              singlet / triplet and use dex_matvec_sym_device */
 } dex_solve_opts;
 
 void dex_solve_opts_default(dex_solve_opts *o);
@@ -141,9 +153,9 @@ int dex_gplmr_eig_device(dex_ctx *ctx, void *x_dev, double *e, int32_t m, int32_
 
 /* Seeded Philox4x32-10 complex Gaussian start block on the device: entry i of
  * column c is (g1 + i g2)/sqrt(2) from counter (i, c, stream_id, 0) and key
- * seed; optionally singlet-projected, X <- (X + X^T)/2 per column. */
+ * seed; symmetry +1 / -1 then projects each column, X <- (X +- X^T)/2. */
 int dex_fill_gaussian_device(dex_ctx *ctx, void *x_dev, int32_t ncols, uint64_t seed,
-                             uint32_t stream_id, int32_t singlet);
+                             uint32_t stream_id, int32_t symmetry);
 int dex_sync(dex_ctx *ctx);
 
 /* Residual norms ||H x_j - E_j x_j|| / ||x_j|| recomputed in FP64 from scratch. */

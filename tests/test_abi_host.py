@@ -88,3 +88,32 @@ def test_host_pencil_matches_lapack_zggev(k, nev, seed):
         assert np.linalg.norm(y) == pytest.approx(1.0, rel=1e-12)
         r = kp @ y - mu[j] * (mp @ y)
         assert np.linalg.norm(r) < 1e-9 * np.linalg.norm(kp, 2)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_host_pencil_degenerate_clusters(seed):
+    """Exactly repeated eigenvalues (3-fold clusters, as O_h symmetry produces):
+    regression for a QR shift that lost the split of a 2 x 2 block to cancellation."""
+    lib = _lib.load()
+    rng = np.random.default_rng(100 + seed)
+    n, k, nev = 300, 32, 8
+    lam = rng.standard_normal(n // 3) + 1j * rng.standard_normal(n // 3)
+    d = np.repeat(lam, 3)
+    v = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a = v @ np.diag(d) @ np.linalg.inv(v)
+    # a basis containing near-converged eigenvectors of the clusters
+    q = np.concatenate([v[:, :9] + 1e-9 * rng.standard_normal((n, 9)),
+                        rng.standard_normal((n, k - 9)) + 0j], axis=1)
+    q, _ = np.linalg.qr(q)
+    sigma = lam[0] + 0.01
+    b = a @ q - sigma * q
+    kp = np.asfortranarray(b.conj().T @ b)
+    mp = np.asfortranarray(b.conj().T @ q)
+    coords = np.zeros((k, nev), dtype=np.complex128, order="F")
+    mu = np.zeros(nev, dtype=np.complex128)
+    st = lib.dex_host_solve_gevp(k, kp.ctypes.data_as(_dp), mp.ctypes.data_as(_dp), k, nev,
+                                 coords.ctypes.data_as(_dp), mu.ctypes.data_as(_dp))
+    assert st == 0
+    w = sla.eig(kp, mp, right=False)
+    ref = np.sort(np.abs(w[np.argsort(np.abs(w))[:nev]]))
+    np.testing.assert_allclose(np.sort(np.abs(mu)), ref, rtol=1e-7)

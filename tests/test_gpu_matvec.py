@@ -143,3 +143,41 @@ def test_dots_match_numpy(dex):
         k = np.asfortranarray(np.stack([_rand(g.d, 30 + j) for j in range(5)], axis=1))
         out = g.mydotsub(b, k, g.d, 3, 5)
     np.testing.assert_allclose(out, b.conj().T @ k, rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("n,sym", [(3, 1), (5, -1), (10, 1), (10, -1)])
+def test_exchange_symmetric_matvec_matches_general(dex, n, sym):
+    """dex_matvec_sym_device on X = sym X^T agrees with the general HAMMULT and
+    its output is exactly (anti)symmetric."""
+    print(f"case: sym matvec N={n} sym={sym}", flush=True)
+    with dex.HeliumGrid(n, 0.4, 0.35) as g:
+        x = torch.randn(g.d, dtype=torch.complex128, device="cuda")
+        xm = x.view(g.m, g.m)
+        x = (0.5 * (xm + sym * xm.t())).contiguous().view(-1)
+        y = torch.empty_like(x)
+        ys = torch.empty_like(x)
+        g.matvec_device(x, y)
+        g.matvec_device(x, ys, symmetry=sym)
+        g.sync()
+        scale = y.abs().max().item()
+        assert (ys - y).abs().max().item() <= 1e-13 * scale
+        ysm = ys.view(g.m, g.m)
+        assert torch.equal(ysm, sym * ysm.t())
+
+
+def test_symmetric_matvec_fp32(dex):
+    print("case: sym matvec fp32 N=10", flush=True)
+    with dex.HeliumGrid(10, 0.4, 0.35, precision="fp32") as g:
+        x = torch.randn(g.d, dtype=torch.complex128, device="cuda")
+        xm = x.view(g.m, g.m)
+        x = (0.5 * (xm + xm.t())).contiguous().view(-1)
+        x32 = x.to(torch.complex64)
+        y32 = torch.empty_like(x32)
+        g.matvec_device(x32, y32, single=True, symmetry=1)
+        g.sync()
+    with dex.HeliumGrid(10, 0.4, 0.35) as g:
+        y = torch.empty_like(x)
+        g.matvec_device(x, y)
+        g.sync()
+    err = (y32.to(torch.complex128) - y).abs().max().item() / y.abs().max().item()
+    assert 1e-12 < err < 2e-6
