@@ -126,7 +126,10 @@ struct Block {
   int size() const { return int(cols.size()); }
 };
 
-ColDesc plain(const Dbl2 *a) { return ColDesc{a, nullptr, 0.0, 0.0}; }
+ColDesc plain(const Dbl2 *a) { return ColDesc{a, nullptr, 0.0, 0.0, -1, 0}; }
+ColDesc shifted(const Dbl2 *a, const Dbl2 *b, cd sig) {
+  return ColDesc{a, b, sig.real(), sig.imag(), -1, 0};
+}
 
 int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
   if (int(ds.size()) > kMaxDesc) return fail(DEX_EINVAL, "too many block columns");
@@ -135,29 +138,65 @@ int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
   return 0;
 }
 
-// G = L^H R (kl x kr, column-major) on the host.
+// Runs one Gram launch (kl <= 64, kr <= 128) and copies the kl x kr result
+// into g (leading dimension ldg) at (i0, j0).  With share > 0 the skipped lower
+// blocks of the Hermitian share x share part are filled by conjugate symmetry.
+int gram_launch_copy(dex_ctx *c, const std::vector<ColDesc> &ds, int kl, int kr, int share,
+                     std::vector<cd> &g, int ldg, int i0, int j0) {
+  if (int st = upload_descs(c, ds)) return st;
+  DEX_HIP(dex::launch_gram(c->d_desc, kl, kr, share, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
+                           c->d_gram_out, c->stream));
+  std::vector<cd> part(size_t(kl) * kr);
+  DEX_HIP(hipMemcpyAsync(part.data(), c->d_gram_out, part.size() * sizeof(Dbl2), hipMemcpyDeviceToHost,
+                         c->stream));
+  DEX_HIP(hipStreamSynchronize(c->stream));
+  for (int j = 0; j < kr; ++j)
+    for (int i = 0; i < kl; ++i) {
+      cd v = part[i + size_t(kl) * j];
+      if (share > 0 && i < share && j < share && (i / 16) > (j / 16)) v = std::conj(part[j + size_t(kl) * i]);
+      g[(i0 + i) + size_t(ldg) * (j0 + j)] = v;
+    }
+  return 0;
+}
+
+// G = L^H R (kl x kr, column-major) on the host; any sizes (chunked launches).
 int gram(dex_ctx *c, const std::vector<ColDesc> &l, const std::vector<ColDesc> &r,
          std::vector<cd> &g) {
   const int kl = int(l.size()), kr = int(r.size());
   g.assign(size_t(kl) * kr, 0.0);
-  if (kl == 0 || kr == 0) return 0;
-  // the kernel takes at most 64 x 128 outputs per launch
   for (int i0 = 0; i0 < kl; i0 += 64)
     for (int j0 = 0; j0 < kr; j0 += 128) {
       const int bl = std::min(64, kl - i0), br = std::min(128, kr - j0);
       std::vector<ColDesc> ds(l.begin() + i0, l.begin() + i0 + bl);
       ds.insert(ds.end(), r.begin() + j0, r.begin() + j0 + br);
-      if (int st = upload_descs(c, ds)) return st;
-      DEX_HIP(dex::launch_gram(c->d_desc, bl, br, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
-                               c->d_gram_out, c->stream));
-      std::vector<cd> part(size_t(bl) * br);
-      DEX_HIP(hipMemcpyAsync(part.data(), c->d_gram_out, part.size() * sizeof(Dbl2),
-                             hipMemcpyDeviceToHost, c->stream));
-      DEX_HIP(hipStreamSynchronize(c->stream));
-      for (int j = 0; j < br; ++j)
-        for (int i = 0; i < bl; ++i) g[(i0 + i) + size_t(kl) * (j0 + j)] = part[i + size_t(bl) * j];
+      if (int st = gram_launch_copy(c, ds, bl, br, 0, g, kl, i0, j0)) return st;
     }
   return 0;
+}
+
+// G = X^H X for k <= 64 columns: every column is read once.
+int gram_self(dex_ctx *c, const std::vector<ColDesc> &x, std::vector<cd> &g) {
+  const int k = int(x.size());
+  if (k > 64) return gram(c, x, x, g);
+  g.assign(size_t(k) * k, 0.0);
+  return gram_launch_copy(c, x, k, k, k, g, k, 0, 0);
+}
+
+// [K | M] = B^H [B | Q] with B = AQ - sigma Q (nq <= 64): AQ and Q are read once.
+int gram_harmonic(dex_ctx *c, const std::vector<Dbl2 *> &q, const std::vector<Dbl2 *> &aq, cd sigma,
+                  std::vector<cd> &g) {
+  const int nq = int(q.size());
+  if (nq < 1 || nq > 64) return fail(DEX_EINVAL, "projected basis must have 1..64 columns");
+  const int ma = (nq + 15) / 16;
+  std::vector<ColDesc> ds;
+  for (int j = 0; j < nq; ++j) {
+    ColDesc dsc = shifted(aq[j], q[j], sigma);
+    dsc.bslot = 16 * ma + j;  // raw Q_j lands in R column nq + j
+    ds.push_back(dsc);
+  }
+  for (int j = 0; j < nq; ++j) ds.push_back(ColDesc{nullptr, nullptr, 0, 0, -1, 0});
+  g.assign(size_t(nq) * 2 * nq, 0.0);
+  return gram_launch_copy(c, ds, nq, 2 * nq, nq, g, nq, 0, 0);
 }
 
 // out_j = sum_i in_i C(i, j) (+ beta_j out_j), row-local, in-place safe.
@@ -186,7 +225,7 @@ int coldots(dex_ctx *c, const std::vector<ColDesc> &a, const std::vector<ColDesc
   const int k = int(a.size());
   std::vector<ColDesc> ds(a);
   if (b.empty()) {
-    for (int j = 0; j < k; ++j) ds.push_back(ColDesc{nullptr, nullptr, 0, 0});
+    for (int j = 0; j < k; ++j) ds.push_back(ColDesc{nullptr, nullptr, 0, 0, -1, 0});
   } else {
     ds.insert(ds.end(), b.begin(), b.end());
   }
@@ -529,7 +568,7 @@ struct Solver {
     const int k = x.size();
     std::vector<cd> g;
     double t0 = now_s();
-    if (int st = gram(c, descs_of(x), descs_of(x), g)) return st;
+    if (int st = gram_self(c, descs_of(x), g)) return st;
     t_gram += now_s() - t0;
     t0 = now_s();
     double asum = 0.0;
@@ -822,13 +861,9 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
       AQ.cols.insert(AQ.cols.end(), b->cols.begin(), b->cols.end());
     const int nq = Q.size();
     if (nq > 64) return fail(DEX_EINVAL, "projected basis larger than 64");
-    std::vector<ColDesc> bl;
-    for (int j = 0; j < nq; ++j) bl.push_back(ColDesc{AQ.cols[j], Q.cols[j], sigma.real(), sigma.imag()});
     std::vector<cd> km;
-    std::vector<ColDesc> rr = bl;
-    for (Dbl2 *p : Q.cols) rr.push_back(plain(p));
     double t0 = now_s();
-    if (int st = gram(c, bl, rr, km)) return st;
+    if (int st = gram_harmonic(c, Q.cols, AQ.cols, sigma, km)) return st;
     S.t_gram += now_s() - t0;
     t0 = now_s();
     std::vector<cd> kp(size_t(nq) * nq), mp(size_t(nq) * nq);
@@ -1022,7 +1057,7 @@ extern "C" int dex_residuals_device(dex_ctx *c, const void *x_dev, const double 
     std::vector<double> aa, bb;
     std::vector<cd> ab;
     // r = AX - E X: coldots of (AX - E X) via the sigma descriptor
-    std::vector<ColDesc> a{ColDesc{ax, xj, e[2 * j], e[2 * j + 1]}};
+    std::vector<ColDesc> a{shifted(ax, xj, cd(e[2 * j], e[2 * j + 1]))};
     std::vector<ColDesc> b{plain(xj)};
     st = coldots(c, a, b, aa, ab, bb);
     if (st) break;

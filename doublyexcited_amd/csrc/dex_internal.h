@@ -35,10 +35,15 @@ struct Dbl2 {
 
 // One column of a tall block as seen by the block kernels: the value at row r
 // is a[r] - sig * b[r] (b may be null).  Used to form A Q - sigma Q on the fly.
+// In the Gram kernel, bslot >= 0 also stores the raw b[r] in that extra
+// column slot (so Q is read once when both B and Q are operands), and a
+// column with a == nullptr is one whose values another column supplies.
 struct ColDesc {
   const Dbl2 *a;
   const Dbl2 *b;
   double sig_re, sig_im;
+  int32_t bslot;
+  int32_t pad;
 };
 
 struct MatvecCoef {
@@ -60,13 +65,16 @@ hipError_t launch_pass2(int nn, bool single, const void *x, const void *u, void 
 hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *vn,
                                  const double *tinv, const MatvecCoef &coef, hipStream_t s);
 
-// G(kl x kr, column-major) = L^H R summed over d rows.  `partial` must hold
-// gram_partial_elems(kl, kr) Dbl2; `descs` is a device array of kl + kr
-// descriptors (L first).
+// G(kl x kr, column-major) = L^H R summed over d rows, kl <= 64, kr <= 128.
+// The first `share` columns of R are the L columns themselves (share <= kl);
+// `descs` holds kl L descriptors then kr - share R descriptors.  With
+// share > 0 the leading share x share block is Hermitian and only its upper
+// 16 x 16 blocks are computed (the rest of that block is left unset).
+// `partial` must hold gram_partial_elems(kl, kr) Dbl2.
 int gram_blocks();
 size_t gram_partial_elems(int kl, int kr);
-hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int64_t d, Dbl2 *partial, Dbl2 *out,
-                       hipStream_t s);
+hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t d, Dbl2 *partial,
+                       Dbl2 *out, hipStream_t s);
 
 // out_j = beta_j out_j + sum_i in_i C(i, j) row by row; in-place aliasing of
 // an output with an input is allowed.  `descs`: nin input descriptors then

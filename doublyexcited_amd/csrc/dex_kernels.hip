@@ -536,16 +536,26 @@ __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
 // threads form a 16 x 16 grid; thread (a, b) owns the outputs
 // (a + 16 m, b + 16 n), m < MA, n < NB, so each LDS row read is shared by many
 // FMAs and consecutive lanes read consecutive columns.  LDS holds kGramRows
-// rows of all kl + kr columns, row-major with one column of padding.
+// rows: L in slots [0, 16 MA), the R columns not shared with L after it.
+// R column j < share is L column j.  For share > 0 the blocks (m, n) with
+// m > n and 16 n < share lie below the diagonal of the Hermitian L^H L part
+// and are skipped.
 template <int MA, int NB>
 __global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict__ descs, int kl,
-                                                       int kr, int64_t d, int64_t rows_per_block,
+                                                       int kr, int share, int64_t d,
+                                                       int64_t rows_per_block,
                                                        double2 *__restrict__ partial) {
   constexpr int LD = 16 * MA + 16 * NB + 1;
   __shared__ double2 tile[kGramRows * LD];
   const int tid = threadIdx.x;
   const int a = tid % 16, b = tid / 16;
-  const int ncol = kl + kr;
+  const int nload = kl + (kr - share);
+  int rslot[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    const int j = b + 16 * n;
+    rslot[n] = j < share ? j : 16 * MA + (j - share);
+  }
   double accr[MA][NB], acci[MA][NB];
 #pragma unroll
   for (int m = 0; m < MA; ++m)
@@ -555,24 +565,42 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict
   const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
   for (int64_t rb = r0; rb < r1; rb += kGramRows) {
     const int nr = int(r1 - rb < kGramRows ? r1 - rb : kGramRows);
-    for (int e = tid; e < kGramRows * ncol; e += kGramThreads) {
+    for (int e = tid; e < kGramRows * nload; e += kGramThreads) {
       const int r = e % kGramRows, c = e / kGramRows;
-      // L columns at [0, 16 MA), R columns at [16 MA, 16 MA + 16 NB)
+      const ColDesc dc = descs[c];
+      if (dc.a == nullptr) continue;  // supplied through another column's bslot
       const int slot = c < kl ? c : 16 * MA + (c - kl);
-      tile[r * LD + slot] = r < nr ? col_value(descs[c], rb + r) : make_double2(0.0, 0.0);
+      if (r < nr) {
+        const int64_t row = rb + r;
+        const double2 va = reinterpret_cast<const double2 *>(dc.a)[row];
+        if (dc.b == nullptr) {
+          tile[r * LD + slot] = va;
+        } else {
+          const double2 vb = reinterpret_cast<const double2 *>(dc.b)[row];
+          double2 o;
+          o.x = va.x - (dc.sig_re * vb.x - dc.sig_im * vb.y);
+          o.y = va.y - (dc.sig_re * vb.y + dc.sig_im * vb.x);
+          tile[r * LD + slot] = o;
+          if (dc.bslot >= 0) tile[r * LD + dc.bslot] = vb;
+        }
+      } else {
+        tile[r * LD + slot] = make_double2(0.0, 0.0);
+        if (dc.b != nullptr && dc.bslot >= 0) tile[r * LD + dc.bslot] = make_double2(0.0, 0.0);
+      }
     }
     __syncthreads();
-#pragma unroll 4
+#pragma unroll 2
     for (int r = 0; r < kGramRows; ++r) {
       double2 lv[MA], rv[NB];
 #pragma unroll
       for (int m = 0; m < MA; ++m) lv[m] = tile[r * LD + a + 16 * m];
 #pragma unroll
-      for (int n = 0; n < NB; ++n) rv[n] = tile[r * LD + 16 * MA + b + 16 * n];
+      for (int n = 0; n < NB; ++n) rv[n] = tile[r * LD + rslot[n]];
 #pragma unroll
       for (int m = 0; m < MA; ++m)
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
+          if (m > n && 16 * n < share) continue;
           accr[m][n] += lv[m].x * rv[n].x + lv[m].y * rv[n].y;
           acci[m][n] += lv[m].x * rv[n].y - lv[m].y * rv[n].x;
         }
@@ -797,40 +825,41 @@ size_t gram_partial_elems(int kl, int kr) { return size_t(kGramBlocks) * kl * kr
 
 namespace {
 template <int MA, int NB>
-void gram_launch(const ColDesc *descs, int kl, int kr, int64_t d, int64_t rpb, double2 *partial,
-                 hipStream_t s) {
-  hipLaunchKernelGGL((k_gram<MA, NB>), dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr, d,
-                     rpb, partial);
+void gram_launch(const ColDesc *descs, int kl, int kr, int share, int64_t d, int64_t rpb,
+                 double2 *partial, hipStream_t s) {
+  hipLaunchKernelGGL((k_gram<MA, NB>), dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr,
+                     share, d, rpb, partial);
 }
 
 template <int MA>
-void gram_launch_nb(int nb, const ColDesc *descs, int kl, int kr, int64_t d, int64_t rpb,
+void gram_launch_nb(int nb, const ColDesc *descs, int kl, int kr, int share, int64_t d, int64_t rpb,
                     double2 *partial, hipStream_t s) {
   switch (nb) {
-    case 1: gram_launch<MA, 1>(descs, kl, kr, d, rpb, partial, s); break;
-    case 2: gram_launch<MA, 2>(descs, kl, kr, d, rpb, partial, s); break;
-    case 3: gram_launch<MA, 3>(descs, kl, kr, d, rpb, partial, s); break;
-    case 4: gram_launch<MA, 4>(descs, kl, kr, d, rpb, partial, s); break;
-    case 5: gram_launch<MA, 5>(descs, kl, kr, d, rpb, partial, s); break;
-    case 6: gram_launch<MA, 6>(descs, kl, kr, d, rpb, partial, s); break;
-    case 7: gram_launch<MA, 7>(descs, kl, kr, d, rpb, partial, s); break;
-    default: gram_launch<MA, 8>(descs, kl, kr, d, rpb, partial, s); break;
+    case 1: gram_launch<MA, 1>(descs, kl, kr, share, d, rpb, partial, s); break;
+    case 2: gram_launch<MA, 2>(descs, kl, kr, share, d, rpb, partial, s); break;
+    case 3: gram_launch<MA, 3>(descs, kl, kr, share, d, rpb, partial, s); break;
+    case 4: gram_launch<MA, 4>(descs, kl, kr, share, d, rpb, partial, s); break;
+    case 5: gram_launch<MA, 5>(descs, kl, kr, share, d, rpb, partial, s); break;
+    case 6: gram_launch<MA, 6>(descs, kl, kr, share, d, rpb, partial, s); break;
+    case 7: gram_launch<MA, 7>(descs, kl, kr, share, d, rpb, partial, s); break;
+    default: gram_launch<MA, 8>(descs, kl, kr, share, d, rpb, partial, s); break;
   }
 }
 }  // namespace
 
-hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int64_t d, Dbl2 *partial, Dbl2 *out,
-                       hipStream_t s) {
+hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t d, Dbl2 *partial,
+                       Dbl2 *out, hipStream_t s) {
   if (kl < 1 || kr < 1 || kl > kGramMaxL || kr > kGramMaxR) return hipErrorInvalidValue;
+  if (share < 0 || share > kl || share > kr) return hipErrorInvalidValue;
   int64_t rpb = (d + kGramBlocks - 1) / kGramBlocks;
   rpb = (rpb + kGramRows - 1) / kGramRows * kGramRows;
   const int ma = (kl + 15) / 16, nb = (kr + 15) / 16;
   double2 *pp = reinterpret_cast<double2 *>(partial);
   switch (ma) {
-    case 1: gram_launch_nb<1>(nb, descs, kl, kr, d, rpb, pp, s); break;
-    case 2: gram_launch_nb<2>(nb, descs, kl, kr, d, rpb, pp, s); break;
-    case 3: gram_launch_nb<3>(nb, descs, kl, kr, d, rpb, pp, s); break;
-    default: gram_launch_nb<4>(nb, descs, kl, kr, d, rpb, pp, s); break;
+    case 1: gram_launch_nb<1>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
+    case 2: gram_launch_nb<2>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
+    case 3: gram_launch_nb<3>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
+    default: gram_launch_nb<4>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
   }
   const int nelem = kl * kr;
   hipLaunchKernelGGL(k_reduce_partials, dim3((nelem + 255) / 256), dim3(256), 0, s,

@@ -154,3 +154,17 @@ def test_orth_block_identities():
     q2, t2 = orth_block(y, drop=True)
     assert q2.shape[1] == 3
     np.testing.assert_allclose(q2.conj().T @ q2, np.eye(3), atol=1e-10)
+
+
+def test_c_oracle_matches_numpy_oracle():
+    from oracle.c_oracle import CHammult
+
+    for n in (1, 3):
+        h = Hamiltonian(n, 0.5, 0.3)
+        c = CHammult(h)
+        rng = np.random.default_rng(n)
+        x = rng.standard_normal((h.d, 2)) + 1j * rng.standard_normal((h.d, 2))
+        y = c(np.asfortranarray(x))
+        for j in range(2):
+            ref = h.hammult(x[:, j])
+            assert np.max(np.abs(y[:, j] - ref)) <= 1e-14 * np.max(np.abs(ref))
