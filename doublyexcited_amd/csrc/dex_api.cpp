@@ -56,7 +56,6 @@ struct dex_ctx {
   hipStream_t stream = nullptr;
   bool fast64 = false, fast32 = false;
   void *d_u = nullptr;  // matvec intermediate, D complex128 (also used as D complex64)
-  void *d_xs = nullptr, *d_ys = nullptr;  // complex64 staging for FP32 matvecs
   ColDesc *d_desc = nullptr;
   Dbl2 *d_small = nullptr;
   Dbl2 *d_small2 = nullptr;
@@ -77,42 +76,27 @@ int ensure_u(dex_ctx *c) {
   return 0;
 }
 
-int ensure_single_staging(dex_ctx *c) {
-  if (c->d_xs) return 0;
-  DEX_HIP(hipMalloc(&c->d_xs, size_t(c->d) * 8));
-  DEX_HIP(hipMalloc(&c->d_ys, size_t(c->d) * 8));
-  return 0;
-}
-
 // Second pass: the general electron-2 cube pass, or for X = sym X^T (sym = +-1)
 // the transpose-add pass.
-hipError_t second_pass(dex_ctx *c, bool single, const void *x, void *y, int sym) {
+hipError_t second_pass(dex_ctx *c, int io, const void *x, void *y, int sym) {
   if (sym != 0)
-    return dex::launch_symadd(c->nn, single, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, sym, c->stream);
-  return dex::launch_pass2(c->nn, single, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, c->stream);
+    return dex::launch_symadd(c->nn, io, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, sym, c->stream);
+  return dex::launch_pass2(c->nn, io, x, c->d_u, y, c->d_vn, c->d_tinv, c->coef, c->stream);
 }
 
-// One application of H in the requested arithmetic; x, y complex128 on device.
+// One application of H on complex128 device vectors; `single` selects FP32
+// arithmetic (complex64 intermediate, complex128 in and out).
 int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0) {
   if (int st = ensure_u(c)) return st;
   hipStream_t s = c->stream;
-  if (single) {
-    if (!c->fast32) return fail(DEX_EUNSUPPORTED, "no FP32 matvec for this grid");
-    if (int st = ensure_single_staging(c)) return st;
-    DEX_HIP(dex::launch_to_single(x, c->d_xs, c->d, s));
+  const bool fast = single ? c->fast32 : c->fast64;
+  if (single && !fast) return fail(DEX_EUNSUPPORTED, "no FP32 matvec for this grid");
+  if (fast) {
+    const int io = single ? dex::kIoMixed : dex::kIoF64;
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
-    DEX_HIP(dex::launch_pass1(c->nn, true, c->d_xs, c->d_u, s));
+    DEX_HIP(dex::launch_pass1(c->nn, io, x, c->d_u, s));
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
-    DEX_HIP(second_pass(c, true, c->d_xs, c->d_ys, sym));
-    if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
-    DEX_HIP(dex::launch_to_double(c->d_ys, y, c->d, s));
-    return 0;
-  }
-  if (c->fast64) {
-    if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
-    DEX_HIP(dex::launch_pass1(c->nn, false, x, c->d_u, s));
-    if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
-    DEX_HIP(second_pass(c, false, x, y, sym));
+    DEX_HIP(second_pass(c, io, x, y, sym));
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
     return 0;
   }
@@ -370,7 +354,7 @@ int dex_ctx_destroy(dex_ctx *c) {
   if (!c) return DEX_OK;
   (void)hipSetDevice(c->p.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (void *ptr : {(void *)c->d_tinv, (void *)c->d_vn, c->d_u, c->d_xs, c->d_ys, (void *)c->d_desc,
+  for (void *ptr : {(void *)c->d_tinv, (void *)c->d_vn, c->d_u, (void *)c->d_desc,
                     (void *)c->d_small, (void *)c->d_small2, (void *)c->d_gram_out, c->d_partial,
                     (void *)c->d_dots})
     if (ptr) (void)hipFree(ptr);
@@ -442,9 +426,9 @@ int dex_matvec_sym_device(dex_ctx *c, const void *in, void *out, int32_t howmany
       const char *xi = static_cast<const char *>(in) + size_t(j) * c->d * 8;
       char *yo = static_cast<char *>(out) + size_t(j) * c->d * 8;
       if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], c->stream));
-      DEX_HIP(dex::launch_pass1(c->nn, true, xi, c->d_u, c->stream));
+      DEX_HIP(dex::launch_pass1(c->nn, dex::kIoF32, xi, c->d_u, c->stream));
       if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], c->stream));
-      DEX_HIP(second_pass(c, true, xi, yo, symmetry));
+      DEX_HIP(second_pass(c, dex::kIoF32, xi, yo, symmetry));
       if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], c->stream));
     } else {
       const Dbl2 *xi = static_cast<const Dbl2 *>(in) + size_t(j) * c->d;
@@ -947,11 +931,10 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
 
 extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns) {
   if (!c) return -1;
-  // X plus (5 + 2 NS) NEV work vectors, the matvec intermediate and staging.
+  // The caller's block X (NEV), the solver's (6 + 2 NS) NEV work vectors and
+  // the matvec intermediate U.
   const int64_t vec = c->d * 16;
-  int64_t total = int64_t(6 + 2 * ns) * nev * vec + vec;
-  if (c->p.precision == DEX_FP32) total += 2 * c->d * 8;
-  return total;
+  return int64_t(7 + 2 * ns) * nev * vec + vec;
 }
 
 extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,

@@ -56,9 +56,14 @@ struct MatvecCoef {
 bool fast_matvec_supported(int nn, bool single);
 bool generic_matvec_supported(int nn);
 
+// Arithmetic / storage of a matvec: FP64 throughout; FP32 with complex64 X
+// and Y; FP32 arithmetic reading complex128 X and writing complex128 Y (the
+// intermediate U is complex64 in both FP32 forms).
+enum { kIoF64 = 0, kIoF32 = 1, kIoMixed = 2 };
+
 // Fast path: U = K1 X (electron 1), then Y = s2 (U + K2 X) + c1 Vr X.
-hipError_t launch_pass1(int nn, bool single, const void *x, void *u, hipStream_t s);
-hipError_t launch_pass2(int nn, bool single, const void *x, const void *u, void *y,
+hipError_t launch_pass1(int nn, int io, const void *x, void *u, hipStream_t s);
+hipError_t launch_pass2(int nn, int io, const void *x, const void *u, void *y,
                         const double *vn, const double *tinv, const MatvecCoef &coef,
                         hipStream_t s);
 // Generic FP64 path for any odd nn <= 27: six axis passes plus the potential.
@@ -95,7 +100,7 @@ hipError_t launch_fill_gaussian(Dbl2 *x, int64_t n, uint64_t seed, uint32_t col,
 // Exchange projection X <- (X + sign X^T)/2 in the M x M view (in place).
 hipError_t launch_sym_project(Dbl2 *x, int64_t m, int sign, hipStream_t s);
 // Second matvec pass for X = sign X^T: Y = s2 (U + sign U^T) + c1 Vr o (X + sign X^T)/2.
-hipError_t launch_symadd(int nn, bool single, const void *x, const void *u, void *y,
+hipError_t launch_symadd(int nn, int io, const void *x, const void *u, void *y,
                          const double *vn, const double *tinv, const MatvecCoef &coef, int sign,
                          hipStream_t s);
 hipError_t launch_to_single(const Dbl2 *x, void *xs, int64_t n, hipStream_t s);

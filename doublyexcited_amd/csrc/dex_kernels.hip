@@ -112,11 +112,13 @@ __device__ __forceinline__ void cube_apply(const R *__restrict__ a, R *__restric
   __syncthreads();
 }
 
-// Pass 1: one workgroup per electron-2 index q; its electron-1 cube is
-// contiguous.  U = tau3D X on that cube (no phase, no potential).
-template <int NN, typename R>
-__global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(
-    const typename Cplx<R>::type *__restrict__ x, typename Cplx<R>::type *__restrict__ u) {
+// Pass 1: electron-1 cubes (contiguous, one per electron-2 index q).
+// U = tau3D X on each cube (no phase, no potential).  Workgroups are
+// persistent: while a workgroup contracts cube q in LDS, the loads of its next
+// cube (q + gridDim.x) are already in flight into registers.
+template <int NN, typename R, typename XT>
+__global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(const XT *__restrict__ x,
+                                                              typename Cplx<R>::type *__restrict__ u) {
   using C = typename Cplx<R>::type;
   constexpr int NT = cube_threads<NN>();
   constexpr int N3 = NN * NN * NN;
@@ -125,41 +127,56 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(
   R *a = reinterpret_cast<R *>(smem);
   R *b = a + N3;
   const int tid = threadIdx.x;
-  const int64_t q = blockIdx.x;
-  const C *xq = x + q * N3;
-  C *uq = u + q * N3;
-  R keep[PER];
+  XT pre[PER];
+  int64_t q = blockIdx.x;
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
     const int e = tid + t * NT;
-    if (e < N3) {
-      const C v = xq[e];
-      a[e] = v.x;
-      keep[t] = v.y;
-    }
+    if (e < N3) pre[t] = x[q * N3 + e];
   }
-  __syncthreads();
-  cube_apply<NN, NT, R>(a, b, tid);
+  for (; q < N3; q += gridDim.x) {
+    R keep[PER];
 #pragma unroll
-  for (int t = 0; t < PER; ++t) {
-    const int e = tid + t * NT;
-    if (e < N3) {
-      const R re = b[e];
-      a[e] = keep[t];
-      keep[t] = re;
+    for (int t = 0; t < PER; ++t) {
+      const int e = tid + t * NT;
+      if (e < N3) {
+        a[e] = R(pre[t].x);
+        keep[t] = R(pre[t].y);
+      }
     }
-  }
-  __syncthreads();
-  cube_apply<NN, NT, R>(a, b, tid);
+    __syncthreads();
+    const int64_t qn = q + gridDim.x;
+    if (qn < N3) {
 #pragma unroll
-  for (int t = 0; t < PER; ++t) {
-    const int e = tid + t * NT;
-    if (e < N3) {
-      C o;
-      o.x = keep[t];
-      o.y = b[e];
-      uq[e] = o;
+      for (int t = 0; t < PER; ++t) {
+        const int e = tid + t * NT;
+        if (e < N3) pre[t] = x[qn * N3 + e];
+      }
     }
+    cube_apply<NN, NT, R>(a, b, tid);
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = tid + t * NT;
+      if (e < N3) {
+        const R re = b[e];
+        a[e] = keep[t];
+        keep[t] = re;
+      }
+    }
+    __syncthreads();
+    cube_apply<NN, NT, R>(a, b, tid);
+    C *uq = u + q * N3;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = tid + t * NT;
+      if (e < N3) {
+        C o;
+        o.x = keep[t];
+        o.y = b[e];
+        uq[e] = o;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -174,11 +191,10 @@ __device__ __forceinline__ int64_t xcd_group_map(int64_t b, int64_t m) {
   return 8 * (x + 8 * (s / 8)) + (s % 8);
 }
 
-template <int NN, typename R>
+template <int NN, typename R, typename XT, typename YT>
 __global__ __launch_bounds__(cube_threads<NN>()) void k_pass2(
-    const typename Cplx<R>::type *__restrict__ x, const typename Cplx<R>::type *__restrict__ u,
-    typename Cplx<R>::type *__restrict__ y, const double *__restrict__ vn,
-    const double *__restrict__ tinv, MatvecCoef cf) {
+    const XT *__restrict__ x, const typename Cplx<R>::type *__restrict__ u, YT *__restrict__ y,
+    const double *__restrict__ vn, const double *__restrict__ tinv, MatvecCoef cf) {
   using C = typename Cplx<R>::type;
   constexpr int NT = cube_threads<NN>();
   constexpr int N3 = NN * NN * NN;
@@ -195,9 +211,9 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass2(
   for (int t = 0; t < PER; ++t) {
     const int e = tid + t * NT;
     if (e < N3) {
-      const C v = x[p + M * e];
-      a[e] = v.x;
-      keep[t] = v.y;
+      const XT v = x[p + M * e];
+      a[e] = R(v.x);
+      keep[t] = R(v.y);
     }
   }
   __syncthreads();
@@ -225,10 +241,10 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass2(
       const int ti = (iq - ip + NN) + L * ((jq - jp + NN) + L * (kq - kp + NN));
       const R vr = R(vnp + vn[e] + tinv[ti] * cf.inv_d3);
       const C uv = u[p + M * e];
-      const C xv = x[p + M * e];
+      const XT xv = x[p + M * e];
       const R sr = uv.x + keep[t], si = uv.y + b[e];
-      const R vxr = vr * xv.x, vxi = vr * xv.y;
-      C o;
+      const R vxr = vr * R(xv.x), vxi = vr * R(xv.y);
+      YT o;
       o.x = s2r * sr - s2i * si + c1r * vxr - c1i * vxi;
       o.y = s2r * si + s2i * sr + c1r * vxi + c1i * vxr;
       y[p + M * e] = o;
@@ -247,17 +263,18 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass2(
 // so the output is exactly (anti)symmetric.
 constexpr int kSymT = 32;
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_symadd(const typename Cplx<R>::type *__restrict__ x,
+template <typename R, typename XT, typename YT>
+__global__ __launch_bounds__(256) void k_symadd(const XT *__restrict__ x,
                                                 const typename Cplx<R>::type *__restrict__ u,
-                                                typename Cplx<R>::type *__restrict__ y, int64_t m,
-                                                int nn, const double *__restrict__ vn,
+                                                YT *__restrict__ y, int64_t m, int nn,
+                                                const double *__restrict__ vn,
                                                 const double *__restrict__ tinv, MatvecCoef cf,
                                                 int sign) {
   using C = typename Cplx<R>::type;
   const int ta = blockIdx.x, tb = blockIdx.y;
   if (ta > tb) return;
-  __shared__ C su[kSymT][kSymT + 1], sx[kSymT][kSymT + 1], sy[kSymT][kSymT + 1];
+  __shared__ C su[kSymT][kSymT + 1], sx[kSymT][kSymT + 1];
+  __shared__ YT sy[kSymT][kSymT + 1];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int64_t pa = int64_t(ta) * kSymT, qb = int64_t(tb) * kSymT;
   // mirror tile: su[r][c] = U[q + m p] with q = qb + c, p = pa + r
@@ -265,7 +282,11 @@ __global__ __launch_bounds__(256) void k_symadd(const typename Cplx<R>::type *__
     const int64_t q = qb + tx, p = pa + r;
     if (q < m && p < m) {
       su[r][tx] = u[q + m * p];
-      sx[r][tx] = x[q + m * p];
+      const XT xv = x[q + m * p];
+      C xc;
+      xc.x = R(xv.x);
+      xc.y = R(xv.y);
+      sx[r][tx] = xc;
     }
   }
   __syncthreads();
@@ -277,16 +298,16 @@ __global__ __launch_bounds__(256) void k_symadd(const typename Cplx<R>::type *__
     const int64_t p = pa + tx, q = qb + r;  // main element (p, q) at p + m q
     if (p < m && q < m) {
       const C uv = u[p + m * q];
-      const C xv = x[p + m * q];
+      const XT xv = x[p + m * q];
       const C um = su[tx][r], xm = sx[tx][r];  // U[q + m p], X[q + m p]
       const R sr = uv.x + sg * um.x, si = uv.y + sg * um.y;
-      const R xr = R(0.5) * (xv.x + sg * xm.x), xi = R(0.5) * (xv.y + sg * xm.y);
+      const R xr = R(0.5) * (R(xv.x) + sg * xm.x), xi = R(0.5) * (R(xv.y) + sg * xm.y);
       const int ip = int(p % nn), jp = int((p / nn) % nn), kp = int(p / (nn * nn));
       const int iq = int(q % nn), jq = int((q / nn) % nn), kq = int(q / (nn * nn));
       const int ti = (iq - ip + nn) + L * ((jq - jp + nn) + L * (kq - kp + nn));
       const R vr = R(vn[p] + vn[q] + tinv[ti] * cf.inv_d3);
       const R vxr = vr * xr, vxi = vr * xi;
-      C o;
+      YT o;
       o.x = s2r * sr - s2i * si + c1r * vxr - c1i * vxi;
       o.y = s2r * si + s2i * sr + c1r * vxi + c1i * vxr;
       sy[tx][r] = o;
@@ -298,8 +319,8 @@ __global__ __launch_bounds__(256) void k_symadd(const typename Cplx<R>::type *__
   for (int r = ty; r < kSymT; r += 8) {
     const int64_t q = qb + tx, p = pa + r;
     if (q < m && p < m && (!diag || r < tx)) {
-      const C o = sy[r][tx];
-      C w;
+      const YT o = sy[r][tx];
+      YT w;
       w.x = sg * o.x;
       w.y = sg * o.y;
       y[q + m * p] = w;
@@ -372,24 +393,36 @@ hipError_t matvec_generic_nn(const double2 *x, double2 *y, const double *vn, con
   return hipGetLastError();
 }
 
-template <int NN, typename R>
+template <int NN, typename R, typename XT>
 hipError_t pass1_nn(const void *x, void *u, hipStream_t s) {
   using C = typename Cplx<R>::type;
   constexpr int N3 = NN * NN * NN;
   const size_t lds = size_t(2) * N3 * sizeof(R);
   static bool attr = false;
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass1<NN, R>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_pass1<NN, R>), dim3(N3), dim3(cube_threads<NN>()), lds, s,
-                     static_cast<const C *>(x), static_cast<C *>(u));
+  // persistent grid: as many workgroups as fit on the device at once
+  static int grid = 0;
+  if (grid == 0) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
+                                                     cube_threads<NN>(), lds) != hipSuccess || per < 1)
+      per = 1;
+    grid = cus * per < N3 ? cus * per : N3;
+  }
+  hipLaunchKernelGGL((k_pass1<NN, R, XT>), dim3(grid), dim3(cube_threads<NN>()), lds, s,
+                     static_cast<const XT *>(x), static_cast<C *>(u));
   return hipGetLastError();
 }
 
-template <int NN, typename R>
+template <int NN, typename R, typename XT, typename YT>
 hipError_t pass2_nn(const void *x, const void *u, void *y, const double *vn, const double *tinv,
                     const MatvecCoef &cf, hipStream_t s) {
   using C = typename Cplx<R>::type;
@@ -397,13 +430,13 @@ hipError_t pass2_nn(const void *x, const void *u, void *y, const double *vn, con
   const size_t lds = size_t(2) * N3 * sizeof(R);
   static bool attr = false;
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass2<NN, R>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass2<NN, R, XT, YT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_pass2<NN, R>), dim3(N3), dim3(cube_threads<NN>()), lds, s,
-                     static_cast<const C *>(x), static_cast<const C *>(u), static_cast<C *>(y), vn,
+  hipLaunchKernelGGL((k_pass2<NN, R, XT, YT>), dim3(N3), dim3(cube_threads<NN>()), lds, s,
+                     static_cast<const XT *>(x), static_cast<const C *>(u), static_cast<YT *>(y), vn,
                      tinv, cf);
   return hipGetLastError();
 }
@@ -443,15 +476,16 @@ bool generic_matvec_supported(int nn) {
   }
 }
 
-hipError_t launch_pass1(int nn, bool single, const void *x, void *u, hipStream_t s) {
+hipError_t launch_pass1(int nn, int io, const void *x, void *u, hipStream_t s) {
   switch (nn) {
 #define CASE(N)                                                                             \
   case N:                                                                                   \
     if constexpr (fast_fits(N, sizeof(double))) {                                           \
-      if (!single) return pass1_nn<N, double>(x, u, s);                                     \
+      if (io == kIoF64) return pass1_nn<N, double, double2>(x, u, s);                       \
     }                                                                                       \
     if constexpr (fast_fits(N, sizeof(float))) {                                            \
-      if (single) return pass1_nn<N, float>(x, u, s);                                       \
+      if (io == kIoF32) return pass1_nn<N, float, float2>(x, u, s);                         \
+      if (io == kIoMixed) return pass1_nn<N, float, double2>(x, u, s);                      \
     }                                                                                       \
     return hipErrorInvalidValue;
     DEX_NN_LIST(CASE)
@@ -461,18 +495,19 @@ hipError_t launch_pass1(int nn, bool single, const void *x, void *u, hipStream_t
   }
 }
 
-hipError_t launch_pass2(int nn, bool single, const void *x, const void *u, void *y,
+hipError_t launch_pass2(int nn, int io, const void *x, const void *u, void *y,
                         const double *vn, const double *tinv, const MatvecCoef &coef,
                         hipStream_t s) {
   switch (nn) {
-#define CASE(N)                                                                             \
-  case N:                                                                                   \
-    if constexpr (fast_fits(N, sizeof(double))) {                                           \
-      if (!single) return pass2_nn<N, double>(x, u, y, vn, tinv, coef, s);                  \
-    }                                                                                       \
-    if constexpr (fast_fits(N, sizeof(float))) {                                            \
-      if (single) return pass2_nn<N, float>(x, u, y, vn, tinv, coef, s);                    \
-    }                                                                                       \
+#define CASE(N)                                                                                   \
+  case N:                                                                                         \
+    if constexpr (fast_fits(N, sizeof(double))) {                                                 \
+      if (io == kIoF64) return pass2_nn<N, double, double2, double2>(x, u, y, vn, tinv, coef, s); \
+    }                                                                                             \
+    if constexpr (fast_fits(N, sizeof(float))) {                                                  \
+      if (io == kIoF32) return pass2_nn<N, float, float2, float2>(x, u, y, vn, tinv, coef, s);    \
+      if (io == kIoMixed) return pass2_nn<N, float, double2, double2>(x, u, y, vn, tinv, coef, s); \
+    }                                                                                             \
     return hipErrorInvalidValue;
     DEX_NN_LIST(CASE)
 #undef CASE
@@ -481,19 +516,23 @@ hipError_t launch_pass2(int nn, bool single, const void *x, const void *u, void 
   }
 }
 
-hipError_t launch_symadd(int nn, bool single, const void *x, const void *u, void *y,
+hipError_t launch_symadd(int nn, int io, const void *x, const void *u, void *y,
                          const double *vn, const double *tinv, const MatvecCoef &coef, int sign,
                          hipStream_t s) {
   const int64_t m = int64_t(nn) * nn * nn;
   const unsigned nt = unsigned((m + kSymT - 1) / kSymT);
-  if (single) {
-    hipLaunchKernelGGL(k_symadd<float>, dim3(nt, nt), dim3(kSymT, 8), 0, s,
-                       static_cast<const float2 *>(x), static_cast<const float2 *>(u),
-                       static_cast<float2 *>(y), m, nn, vn, tinv, coef, sign);
+  const dim3 g(nt, nt), b(kSymT, 8);
+  if (io == kIoF32) {
+    hipLaunchKernelGGL((k_symadd<float, float2, float2>), g, b, 0, s, static_cast<const float2 *>(x),
+                       static_cast<const float2 *>(u), static_cast<float2 *>(y), m, nn, vn, tinv, coef, sign);
+  } else if (io == kIoMixed) {
+    hipLaunchKernelGGL((k_symadd<float, double2, double2>), g, b, 0, s, static_cast<const double2 *>(x),
+                       static_cast<const float2 *>(u), static_cast<double2 *>(y), m, nn, vn, tinv, coef,
+                       sign);
   } else {
-    hipLaunchKernelGGL(k_symadd<double>, dim3(nt, nt), dim3(kSymT, 8), 0, s,
-                       static_cast<const double2 *>(x), static_cast<const double2 *>(u),
-                       static_cast<double2 *>(y), m, nn, vn, tinv, coef, sign);
+    hipLaunchKernelGGL((k_symadd<double, double2, double2>), g, b, 0, s, static_cast<const double2 *>(x),
+                       static_cast<const double2 *>(u), static_cast<double2 *>(y), m, nn, vn, tinv, coef,
+                       sign);
   }
   return hipGetLastError();
 }

@@ -1,0 +1,187 @@
+"""Continuous-batching service for resonance solves on one GPU.
+
+Requests (grid N, spacing, scaling angle theta, complex shift sigma, block
+size, tolerance, precision) arrive over time.  The scheduler admits a request
+as soon as (a) a worker slot is free and (b) its device footprint fits, together
+with every solve already on the GPU, under the HBM cap.  Each admitted solve owns
+a context (its own HIP stream and buffers), so solves overlap on the device.
+Results carry arrival->completion latency.  A request that fails, whether it is
+rejected at admission or errors in the solver, yields an error result; it never
+stops the service.
+
+The reference runs one solve per process (`SAVE`d scratch, hammod.F90:153-159
+and :872-896); per-request contexts replace that restriction.
+"""
+from __future__ import annotations
+
+import dataclasses
+import heapq
+import threading
+import time
+
+import numpy as np
+
+
+@dataclasses.dataclass(order=True)
+class Request:
+    arrival: float
+    rid: int
+    n: int = dataclasses.field(compare=False)          # half-width N, NN = 2N + 1
+    delta: float = dataclasses.field(compare=False)
+    theta: float = dataclasses.field(compare=False)
+    sigma: complex = dataclasses.field(compare=False)
+    nev: int = dataclasses.field(compare=False)
+    tol: float = dataclasses.field(compare=False)
+    precision: str = dataclasses.field(compare=False, default="fp64")
+    maxit: int = dataclasses.field(compare=False, default=3000)
+    seed: int = dataclasses.field(compare=False, default=0)
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    ok: bool
+    e: np.ndarray | None
+    residuals: np.ndarray | None
+    converged_pairs: int
+    iterations: int
+    latency: float           # completion - arrival (s)
+    service: float           # completion - admission (s)
+    error: str | None = None
+
+
+def snap_nn(nn: int) -> int:
+    """Grid points per axis -> the reference's half-width N.  Even NN has no
+    reference counterpart (the nucleus would sit between grid points, a path
+    that STOPs at hammod.F90:310); it is snapped up to NN + 1."""
+    if nn % 2 == 0:
+        nn += 1
+    return (nn - 1) // 2
+
+
+def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1) -> int:
+    """Device bytes a solve holds: the block X, (6 + 2 NS) NEV work vectors and
+    the matvec intermediate (the same for both precisions: the basis is complex128)."""
+    d = (2 * n + 1) ** 6
+    return (7 + 2 * ns) * nev * 16 * d + 16 * d
+
+
+def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),
+                   delta_range=(0.35, 0.6), theta_range=(0.2, 0.5),
+                   sigma_re=(-0.78, -0.69, -0.60), sigma_im=-0.002, nev_choices=(4, 8, 16),
+                   tol=1e-7, precision="fp64"):
+    """Seeded request stream with Poisson arrivals."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        nn = int(rng.choice(list(nn_choices)))
+        out.append(Request(arrival=t, rid=i, n=snap_nn(nn), delta=float(rng.uniform(*delta_range)),
+                           theta=float(rng.uniform(*theta_range)),
+                           sigma=complex(float(rng.choice(sigma_re)), sigma_im),
+                           nev=int(rng.choice(nev_choices)), tol=tol, precision=precision,
+                           seed=seed * 100003 + i))
+    return out
+
+
+class Server:
+    def __init__(self, hbm_cap_bytes: float = 240e9, slots: int = 4, device: int = 0):
+        self.cap = float(hbm_cap_bytes)
+        self.slots = slots
+        self.device = device
+        self._lock = threading.Lock()
+        self._in_use = 0.0
+        self._free_slots = slots
+        self._cv = threading.Condition(self._lock)
+
+    def _solve(self, req: Request, t_admit: float, t0: float) -> Result:
+        """Runs one request; t_admit and the returned times are relative to t0."""
+        import torch
+
+        import doublyexcited_amd as dex
+
+        try:
+            torch.cuda.set_device(self.device)
+            with dex.HeliumGrid(req.n, req.delta, req.theta, precision=req.precision,
+                                device=self.device) as g:
+                x = torch.empty(g.d * req.nev, dtype=torch.complex128, device=f"cuda:{self.device}")
+                g.fill_gaussian_device(x, req.nev, seed=req.seed, symmetry=1)
+                r = g.gplmr_eig_device(x, req.sigma, req.nev, ns=1, maxit=req.maxit, tol=req.tol,
+                                       symmetry=1, fp32_tol=max(req.tol, 2e-6))
+                res = g.residuals_device(x, r.e)
+                del x
+            done = time.time() - t0
+            conv = int(np.sum(res <= 1.5 * req.tol))
+            return Result(req.rid, True, r.e, res, conv, r.iterations, done - req.arrival,
+                          done - t_admit)
+        except Exception as exc:  # isolate the failure to this request
+            done = time.time() - t0
+            return Result(req.rid, False, None, None, 0, 0, done - req.arrival, done - t_admit,
+                          error=repr(exc))
+
+    def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:
+        """Serves the requests (arrival times relative to the start); returns
+        results in request order."""
+        pending = sorted(requests)
+        results: dict[int, Result] = {}
+        threads: list[threading.Thread] = []
+        t0 = time.time()
+        waiting: list = []  # heap of (arrival, rid, req) that arrived but are not admitted
+        i = 0
+        while i < len(pending) or waiting:
+            now = time.time() - t0
+            while i < len(pending) and (pending[i].arrival <= now or not realtime):
+                heapq.heappush(waiting, (pending[i].arrival, pending[i].rid, pending[i]))
+                i += 1
+            admitted = False
+            with self._cv:
+                if waiting and self._free_slots > 0:
+                    _, _, req = waiting[0]
+                    need = footprint_bytes(req.n, req.nev, req.precision)
+                    if need > self.cap:
+                        heapq.heappop(waiting)
+                        results[req.rid] = Result(req.rid, False, None, None, 0, 0, 0.0, 0.0,
+                                                  error=f"admission: {need/1e9:.1f} GB exceeds the "
+                                                        f"{self.cap/1e9:.0f} GB cap")
+                        admitted = True
+                    elif self._in_use + need <= self.cap:
+                        heapq.heappop(waiting)
+                        self._in_use += need
+                        self._free_slots -= 1
+                        t_admit = time.time() - t0
+
+                        def work(req=req, need=need, t_admit=t_admit):
+                            res = self._solve(req, t_admit, t0)
+                            with self._cv:
+                                results[req.rid] = res
+                                self._in_use -= need
+                                self._free_slots += 1
+                                self._cv.notify_all()
+
+                        th = threading.Thread(target=work, daemon=True)
+                        th.start()
+                        threads.append(th)
+                        admitted = True
+                if not admitted:
+                    timeout = 0.05
+                    if i < len(pending) and realtime:
+                        timeout = max(0.0, min(timeout, pending[i].arrival - (time.time() - t0)))
+                    self._cv.wait(timeout=timeout if timeout > 0 else 0.001)
+        for th in threads:
+            th.join()
+        return [results[r.rid] for r in requests]
+
+
+def summarize(results: list[Result], wall: float) -> dict:
+    lat = [r.latency for r in results if r.ok]
+    pairs = sum(r.converged_pairs for r in results)
+    return {
+        "requests": len(results),
+        "ok": sum(r.ok for r in results),
+        "converged_pairs": pairs,
+        "eigenpairs_per_s": pairs / wall if wall > 0 else 0.0,
+        "p50_latency_s": float(np.percentile(lat, 50)) if lat else None,
+        "p99_latency_s": float(np.percentile(lat, 99)) if lat else None,
+        "wall_s": wall,
+    }

@@ -162,7 +162,8 @@ int dex_sync(dex_ctx *ctx);
 int dex_residuals_device(dex_ctx *ctx, const void *x_dev, const double *e, int32_t nev,
                          double *res_out);
 
-/* Bytes of device memory dex_gplmr_eig needs for (nev, ns) on this context. */
+/* Bytes of device memory a dex_gplmr_eig_device solve of (nev, ns) holds on this
+ * context, the caller's block X included. */
 int64_t dex_gplmr_footprint(const dex_ctx *ctx, int32_t nev, int32_t ns);
 
 /* Host-only set-up and small-matrix routines (no device use), exported so the

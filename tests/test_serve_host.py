@@ -1,0 +1,35 @@
+"""Host-side logic of the serving layer (no GPU): request generation, grid
+snapping, footprints and admission bookkeeping."""
+import numpy as np
+
+from doublyexcited_amd.serve import Request, Server, footprint_bytes, poisson_stream, snap_nn
+
+
+def test_snap_nn_rounds_even_grids_up():
+    assert snap_nn(11) == 5 and snap_nn(12) == 6 and snap_nn(21) == 10 and snap_nn(20) == 10
+
+
+def test_footprint_matches_formula():
+    d = 21 ** 6
+    assert footprint_bytes(10, 16, "fp64") == (7 + 2) * 16 * 16 * d + 16 * d
+    assert footprint_bytes(10, 16, "mixed") == footprint_bytes(10, 16, "fp64")
+    # NN=21, block 16 fits the 240 GB cap alone; two of them do not
+    assert footprint_bytes(10, 16, "fp64") < 240e9 < 2 * footprint_bytes(10, 16, "fp64")
+
+
+def test_poisson_stream_is_seeded_and_in_range():
+    a = poisson_stream(0.5, 20, seed=1005)
+    b = poisson_stream(0.5, 20, seed=1005)
+    assert [(r.arrival, r.n, r.delta, r.theta, r.nev) for r in a] == \
+           [(r.arrival, r.n, r.delta, r.theta, r.nev) for r in b]
+    assert all(5 <= r.n <= 10 for r in a)
+    assert all(0.35 <= r.delta <= 0.6 and 0.2 <= r.theta <= 0.5 for r in a)
+    assert {r.nev for r in a} <= {4, 8, 16}
+    assert np.all(np.diff([r.arrival for r in a]) > 0)
+
+
+def test_oversized_request_is_rejected_without_stopping_the_service():
+    s = Server(hbm_cap_bytes=1e9, slots=2)
+    big = Request(arrival=0.0, rid=0, n=10, delta=0.4, theta=0.3, sigma=-0.7 + 0j, nev=16, tol=1e-7)
+    out = s.run([big], realtime=False)
+    assert not out[0].ok and "admission" in out[0].error
