@@ -774,6 +774,22 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     if (o.refresh_every > 0 && iter > 1 && (iter - 1) % o.refresh_every == 0) {
       if (int st = S.matblock(X, AX)) return st;
     }
+    // active columns: all, or with lock only those still above TOL
+    std::vector<int> active;
+    for (int j = 0; j < nev; ++j)
+      if (!o.lock || resn[j] > tol) active.push_back(j);
+    {
+      Block wa;
+      std::vector<bool> keep(nev, false);
+      for (int j : active) keep[j] = true;
+      for (int j = 0; j < nev; ++j) {
+        if (keep[j]) wa.cols.push_back(W.cols[j]);
+        else S.pool.push_back(W.cols[j]);
+      }
+      W = wa;
+    }
+    std::vector<cd> eact;
+    for (int j : active) eact.push_back(e[j]);
     // CHOLQR(X); AX <- AX R^{-1}  (:88-90)
     if (int st = S.orth(X, &AX, false)) return -st;
     if (X.size() != nev) return fail(DEX_ECHOLESKY, "X lost rank");
@@ -806,7 +822,7 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
         std::vector<cd> cm(size_t(2 * k) * k, 0.0);
         for (int j = 0; j < k; ++j) {
           cm[j + size_t(2 * k) * j] = 1.0;
-          cm[k + j + size_t(2 * k) * j] = -e[j];
+          cm[k + j + size_t(2 * k) * j] = -eact[j];
         }
         std::vector<ColDesc> in = descs_of(aprev);
         for (Dbl2 *p : prev.cols) in.push_back(plain(p));
@@ -861,23 +877,25 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
       return fail(DEX_EGEVP, "SOLVE OF THE PROJECTED EIGENVALUE PROBLEM FAILED");
     S.t_small += now_s() - t0;
     t0 = now_s();
-    // P = W CW + S CS + P CP;  X = X CX + P  (and images) (:298-320)
+    // P = W CW + S CS + P CP;  X = X CX + P  (and images) (:298-320).
+    // The new P keeps the columns of the active set.
     const int nw = W.size(), nsb = Sblk.size(), np = P.size();
+    const int nact = int(active.size());
     // inputs [X W S P], outputs [X, Pnew]
     Block Pn, APn;
-    for (int j = 0; j < np && j < nev; ++j) {
+    for (int j = 0; j < np && j < nact; ++j) {
       Pn.cols.push_back(P.cols[j]);
       APn.cols.push_back(AP.cols[j]);
     }
-    if (int st = S.grow(Pn, nev - Pn.size())) return st;
-    if (int st = S.grow(APn, nev - APn.size())) return st;
-    std::vector<cd> cm(size_t(nq) * 2 * nev, 0.0);
+    if (int st = S.grow(Pn, nact - Pn.size())) return st;
+    if (int st = S.grow(APn, nact - APn.size())) return st;
+    std::vector<cd> cm(size_t(nq) * (nev + nact), 0.0);
     for (int j = 0; j < nev; ++j)
-      for (int i = 0; i < nq; ++i) {
-        const cd v = coords[i + size_t(nq) * j];
-        cm[i + size_t(nq) * j] = v;                              // X_new
-        if (i >= nev) cm[i + size_t(nq) * (nev + j)] = v;        // P_new
-      }
+      for (int i = 0; i < nq; ++i) cm[i + size_t(nq) * j] = coords[i + size_t(nq) * j];  // X_new
+    for (int jj = 0; jj < nact; ++jj) {
+      const int j = active[jj];
+      for (int i = nev; i < nq; ++i) cm[i + size_t(nq) * (nev + jj)] = coords[i + size_t(nq) * j];  // P_new
+    }
     {
       std::vector<Dbl2 *> out = X.cols;
       out.insert(out.end(), Pn.cols.begin(), Pn.cols.end());
@@ -888,7 +906,7 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     }
     (void)nw;
     (void)nsb;
-    for (int j = nev; j < np; ++j) {
+    for (int j = nact; j < np; ++j) {
       S.pool.push_back(P.cols[j]);
       S.pool.push_back(AP.cols[j]);
     }

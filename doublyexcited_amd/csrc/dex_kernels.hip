@@ -656,17 +656,25 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict
     }
 }
 
-__global__ void k_reduce_partials(const double2 *__restrict__ partial, int nparts, int nelem,
-                                  double2 *__restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// Sum of nparts partial results per element: one wavefront per element, each
+// lane summing a strided subset, then a shuffle tree (fixed order).
+__global__ __launch_bounds__(256) void k_reduce_partials(const double2 *__restrict__ partial,
+                                                         int nparts, int nelem,
+                                                         double2 *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= nelem) return;
   double sx = 0.0, sy = 0.0;
-  for (int b = 0; b < nparts; ++b) {
+  for (int b = lane; b < nparts; b += 64) {
     const double2 v = partial[size_t(b) * nelem + e];
     sx += v.x;
     sy += v.y;
   }
-  out[e] = make_double2(sx, sy);
+  for (int off = 32; off > 0; off >>= 1) {
+    sx += __shfl_down(sx, off);
+    sy += __shfl_down(sy, off);
+  }
+  if (lane == 0) out[e] = make_double2(sx, sy);
 }
 
 constexpr int kCombRows = 64;
@@ -775,13 +783,15 @@ __global__ __launch_bounds__(kDotThreads) void k_coldots(const ColDesc *__restri
   }
 }
 
-__global__ void k_reduce_real(const double *__restrict__ partial, int nparts, int nelem,
-                              double *__restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_reduce_real(const double *__restrict__ partial, int nparts,
+                                                     int nelem, double *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= nelem) return;
   double s = 0.0;
-  for (int b = 0; b < nparts; ++b) s += partial[size_t(b) * nelem + e];
-  out[e] = s;
+  for (int b = lane; b < nparts; b += 64) s += partial[size_t(b) * nelem + e];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if (lane == 0) out[e] = s;
 }
 
 // Philox4x32-10 (Salmon et al., SC'11) and Box-Muller.
@@ -901,7 +911,7 @@ hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t 
     default: gram_launch_nb<4>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
   }
   const int nelem = kl * kr;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((nelem + 255) / 256), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_reduce_partials, dim3((nelem + 3) / 4), dim3(256), 0, s,
                      reinterpret_cast<const double2 *>(partial), kGramBlocks, nelem,
                      reinterpret_cast<double2 *>(out));
   return hipGetLastError();
@@ -924,7 +934,7 @@ hipError_t launch_coldots(const ColDesc *descs, int k, int64_t d, double *partia
   if (k < 1 || k > kDotMaxCols) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_coldots, dim3(kDotBlocks), dim3(kDotThreads), 0, s, descs, k, d, partial);
   const int nelem = k * 4;
-  hipLaunchKernelGGL(k_reduce_real, dim3((nelem + 255) / 256), dim3(256), 0, s, partial, kDotBlocks,
+  hipLaunchKernelGGL(k_reduce_real, dim3((nelem + 3) / 4), dim3(256), 0, s, partial, kDotBlocks,
                      nelem, out);
   return hipGetLastError();
 }

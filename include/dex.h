@@ -131,6 +131,9 @@ typedef struct dex_solve_opts {
   double fp32_tol;         /* FP32 contexts: residual target of the FP32-matvec phase */
   int32_t symmetry;        /* 0: general HAMMULT; +1 / -1: project the start block to
                               singlet / triplet and use dex_matvec_sym_device */
+  int32_t lock;            /* 1: build W, S, P only for columns whose residual is above
+                              TOL (all columns stay in X and the projection); 0 (default):
+                              every column, as the reference */
 } dex_solve_opts;
 
 void dex_solve_opts_default(dex_solve_opts *o);

@@ -139,11 +139,16 @@ class GplmrResult:
 
 def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: complex,
               nev: int, ns: int = 1, maxit: int = 100, tol: float = 1e-8,
-              inner_iter: int = 0, callback=None) -> GplmrResult:
+              inner_iter: int = 0, callback=None, lock: bool = False) -> GplmrResult:
     """GPLMR_EIG, gplmr.F90:9-386, with GM_MAX_INNER_ITER in {0, 1}.
 
     ``inner_iter == 1`` applies PRECBLOCK's ``W <- H W`` (gplmr.F90:491-493);
     ``0`` skips the preconditioner (guards at gplmr.F90:92,118,151).
+
+    ``lock`` (not in the reference): search directions W, S and P are built
+    only for columns whose residual is still above ``tol``; every column stays
+    in X and in the projected pencil.  With ``lock=False`` every column
+    contributes directions, as in the reference.
     """
     if ns < 1:
         raise GplmrError("NS must be >= 1")  # R is (NS*NEV, NEV), gplmr.F90:36
@@ -174,20 +179,23 @@ def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: c
     it = 1
     history = []
     while it <= maxit and maxres > tol:
+        active = np.flatnonzero(resnrm > tol) if lock else np.arange(nev)
+        w = w[:, active]
         x, t = orth_block(x, drop=False)  # :88
         ax = ax @ t  # :90
         w = prec(w)
         w = w - x @ crossmult(x, w)  # :101-103
         w, _ = orth_block(w, drop=True)  # :105
         aw = matblock(w)  # :108
-        # S_1 = AW - W E  (:114-116); E pairs with the leading columns
-        ew = e[:w.shape[1]]
+        # S_1 = AW - W E  (:114-116); E pairs with the leading active columns
+        eact = e[active]
+        ew = eact[:w.shape[1]]
         sblocks, asblocks = [], []
         snew = aw - w * ew[None, :]
         for l in range(1, ns + 1):  # :114-172
             if l > 1:
                 prev, aprev = sblocks[-1], asblocks[-1]
-                snew = aprev - prev * e[:prev.shape[1]][None, :]
+                snew = aprev - prev * eact[:prev.shape[1]][None, :]
             snew = prec(snew)
             snew = snew - x @ crossmult(x, snew)  # :127-128
             snew = snew - w @ crossmult(w, snew)  # :129-130
@@ -229,9 +237,9 @@ def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: c
         if p.shape[1] > 0:  # :298-305
             p_new += p @ cp
             ap_new += ap @ cp
-        p, ap = p_new, ap_new
-        x = x @ cx + p  # :316-318
-        ax = ax @ cx + ap
+        x = x @ cx + p_new  # :316-318
+        ax = ax @ cx + ap_new
+        p, ap = p_new[:, active], ap_new[:, active]
         nrm = colnorms(x)  # :324
         x /= nrm[None, :]
         ax /= nrm[None, :]

@@ -123,8 +123,9 @@ def test_reference_logged_singlet_resonance(dex, golden_dir):
     with dex.HeliumGrid(p["N"], p["delta"], p["theta"]) as grid:
         x = torch.empty(grid.d, dtype=torch.complex128, device="cuda")
         grid.fill_gaussian_device(x, 1, seed=2024, symmetry=1)
-        r = grid.gplmr_eig_device(x, complex(-0.7110341829, 0.0313569500), 1, ns=1, maxit=4000,
-                                  tol=1e-9)
+        # NS = 5 is the reference's default GPLMR_NS (input.F90:81), which its logged run used
+        r = grid.gplmr_eig_device(x, complex(-0.7110341829, 0.0313569500), 1, ns=5, maxit=4000,
+                                  tol=1e-9, symmetry=1)
         res = grid.residuals_device(x, r.e)
     assert r.converged, (r.iterations, r.max_residual)
     assert res[0] < 2e-9
