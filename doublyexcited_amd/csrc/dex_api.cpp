@@ -326,7 +326,9 @@ int dex_ctx_create(const dex_params *p, dex_ctx **out) {
   // two-pass kernels exist (used to cross-check the two implementations).
   if (const char *mv = std::getenv("DEX_MATVEC"))
     if (std::strcmp(mv, "generic") == 0) c->fast64 = false;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  // A blocking stream: work on it is ordered after work already queued on the
+  // default stream (where callers such as torch place their kernels) and vice versa.
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
   if (e == hipSuccess) e = hipMalloc(&c->d_tinv, c->tinv_host.size() * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->d_vn, c->vn_host.size() * sizeof(double));
   if (e == hipSuccess)

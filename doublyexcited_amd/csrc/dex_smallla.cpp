@@ -112,19 +112,20 @@ void zheev_jacobi(int n, const cd *a_in, int lda, std::vector<double> &w, std::v
   v.swap(vs);
 }
 
-// Eigenvalues and right eigenvectors of a general complex n x n matrix:
-// Householder reduction to upper Hessenberg form, complex single-shift QR with
-// Wilkinson shifts and deflation to Schur form T = Z^H A Z (Golub & Van Loan,
-// Matrix Computations, 4th ed., Alg. 7.4.2 and 7.5), then back substitution
-// on T and back-transformation by Z.  Returns 0, or 1 if QR did not converge.
-int zgeev_right(int n, const cd *a_in, int lda, std::vector<cd> &lam, std::vector<cd> &vec) {
-  std::vector<cd> h(size_t(n) * n), z(size_t(n) * n, 0.0);
+// Complex Schur form A = Z T Z^H of a general n x n matrix: Householder
+// reduction to upper Hessenberg form, then complex single-shift QR with
+// Wilkinson shifts and deflation (Golub & Van Loan, Matrix Computations, 4th
+// ed., Alg. 7.4.2 and 7.5).  T is returned in h (upper triangular), Z in z.
+// Returns 0, or 1 if QR did not converge.
+static int zschur(int n, const cd *a_in, int lda, std::vector<cd> &h, std::vector<cd> &z, double &anorm) {
+  h.assign(size_t(n) * n, 0.0);
+  z.assign(size_t(n) * n, 0.0);
   auto H = [&](int i, int j) -> cd & { return h[i + size_t(n) * j]; };
   auto Z = [&](int i, int j) -> cd & { return z[i + size_t(n) * j]; };
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) H(i, j) = a_in[i + size_t(lda) * j];
   for (int i = 0; i < n; ++i) Z(i, i) = 1.0;
-  double anorm = 0.0;
+  anorm = 0.0;
   for (size_t i = 0; i < h.size(); ++i) anorm = std::max(anorm, std::abs(h[i]));
   // Hessenberg reduction
   std::vector<cd> u(n);
@@ -273,37 +274,119 @@ int zgeev_right(int n, const cd *a_in, int lda, std::vector<cd> &lam, std::vecto
     kdefl = 0;
     i = l - 1;
   }
-  const double eps = ulp;
-  lam.resize(n);
-  for (int i = 0; i < n; ++i) lam[i] = H(i, i);
-  // eigenvectors of the triangular T, then back-transform
-  vec.assign(size_t(n) * n, 0.0);
-  std::vector<cd> y(n);
+  for (int j = 0; j < n; ++j)
+    for (int r = j + 1; r < n; ++r) H(r, j) = 0.0;
+  return 0;
+}
+
+// Exchanges the diagonal entries k and k + 1 of the Schur form (T, Z) with one
+// plane rotation whose first column is the eigenvector of the 2 x 2 block for
+// T(k+1, k+1) (Golub & Van Loan, Sec. 7.6.2).
+static void schur_swap(int n, std::vector<cd> &t, std::vector<cd> &z, int k) {
+  auto T = [&](int i, int j) -> cd & { return t[i + size_t(n) * j]; };
+  const cd t11 = T(k, k), t22 = T(k + 1, k + 1), t12 = T(k, k + 1);
+  const cd d = t22 - t11;
+  const double r = std::sqrt(std::norm(t12) + std::norm(d));
+  if (r == 0.0) return;  // equal entries, nothing to exchange
+  const cd g1 = t12 / r, g2 = d / r;
+  // G = [[g1, -conj(g2)], [g2, conj(g1)]];  T <- G^H T G,  Z <- Z G
+  for (int j = 0; j < n; ++j) {
+    const cd a = T(k, j), b = T(k + 1, j);
+    T(k, j) = std::conj(g1) * a + std::conj(g2) * b;
+    T(k + 1, j) = -g2 * a + g1 * b;
+  }
+  for (int i = 0; i < n; ++i) {
+    const cd a = T(i, k), b = T(i, k + 1);
+    T(i, k) = a * g1 + b * g2;
+    T(i, k + 1) = -a * std::conj(g2) + b * std::conj(g1);
+    const cd za = z[i + size_t(n) * k], zb = z[i + size_t(n) * (k + 1)];
+    z[i + size_t(n) * k] = za * g1 + zb * g2;
+    z[i + size_t(n) * (k + 1)] = -za * std::conj(g2) + zb * std::conj(g1);
+  }
+  T(k + 1, k) = 0.0;
+  T(k, k) = t22;
+  T(k + 1, k + 1) = t11;
+}
+
+// Right eigenvectors (unit 2-norm) for the leading p diagonal entries of the
+// Schur form, by back substitution on T and multiplication by Z.  Entries of
+// one cluster (contiguous, equal to within tolc relative) are treated as a
+// multiple eigenvalue: each member keeps its own Schur direction instead of
+// dividing by a vanishing difference, so the cluster's vectors span its
+// invariant subspace with an orthonormal basis.
+static void leading_eigvecs(int n, int p, const std::vector<cd> &t, const std::vector<cd> &z, double anorm,
+                            std::vector<cd> &vec) {
+  const double eps = 2.220446049250313e-16, tolc = 1e-9;
+  auto T = [&](int i, int j) -> const cd & { return t[i + size_t(n) * j]; };
+  std::vector<int> cl(p, 0);
+  for (int i = 1; i < p; ++i) {
+    const double sc = std::max(std::abs(T(i, i)), std::abs(T(i - 1, i - 1)));
+    cl[i] = std::abs(T(i, i) - T(i - 1, i - 1)) <= tolc * sc ? cl[i - 1] : cl[i - 1] + 1;
+  }
   const double smin = std::max(eps * anorm, 1e-300);
-  for (int k = 0; k < n; ++k) {
-    for (int i = 0; i < n; ++i) y[i] = 0.0;
+  vec.assign(size_t(n) * p, 0.0);
+  std::vector<cd> y(p);
+  for (int k = 0; k < p; ++k) {
+    std::fill(y.begin(), y.end(), cd(0.0));
     y[k] = 1.0;
     for (int i = k - 1; i >= 0; --i) {
+      if (cl[i] == cl[k]) continue;  // cluster mate: y_i = 0
       cd s = 0.0;
-      for (int j = i + 1; j <= k; ++j) s += H(i, j) * y[j];
-      cd den = H(i, i) - lam[k];
+      for (int j = i + 1; j <= k; ++j) s += T(i, j) * y[j];
+      cd den = T(i, i) - T(k, k);
       if (std::abs(den) < smin) den = smin;
       y[i] = -s / den;
     }
     double nrm = 0.0;
-    for (int i = 0; i < n; ++i) {
+    for (int r = 0; r < n; ++r) {
       cd s = 0.0;
-      for (int j = 0; j <= k; ++j) s += Z(i, j) * y[j];
-      vec[i + size_t(n) * k] = s;
+      for (int j = 0; j <= k; ++j) s += z[r + size_t(n) * j] * y[j];
+      vec[r + size_t(n) * k] = s;
       nrm += std::norm(s);
     }
     nrm = std::sqrt(nrm);
     if (nrm > 0)
-      for (int i = 0; i < n; ++i) vec[i + size_t(n) * k] /= nrm;
+      for (int r = 0; r < n; ++r) vec[r + size_t(n) * k] /= nrm;
   }
+}
+
+// Eigenvalues and right eigenvectors of a general complex n x n matrix.
+int zgeev_right(int n, const cd *a_in, int lda, std::vector<cd> &lam, std::vector<cd> &vec) {
+  std::vector<cd> t, z;
+  double anorm = 0.0;
+  if (zschur(n, a_in, lda, t, z, anorm) != 0) return 1;
+  lam.resize(n);
+  for (int i = 0; i < n; ++i) lam[i] = t[i + size_t(n) * i];
+  leading_eigvecs(n, n, t, z, anorm, vec);
   return 0;
 }
 
+// Selects the p eigenvalues of smallest key (key(lambda) from `key`), moves
+// them, in that order, to the leading positions of the Schur form, and returns
+// them with their eigenvectors.  Returns 0, or 1 if QR did not converge.
+template <typename Key>
+static int select_eigen(int n, const cd *a, int lda, int p, Key key, std::vector<cd> &lam,
+                        std::vector<cd> &vec) {
+  std::vector<cd> t, z;
+  double anorm = 0.0;
+  if (zschur(n, a, lda, t, z, anorm) != 0) return 1;
+  for (int pos = 0; pos < p; ++pos) {
+    int best = pos;
+    double bk = key(t[pos + size_t(n) * pos]);
+    for (int i = pos + 1; i < n; ++i) {
+      const double ki = key(t[i + size_t(n) * i]);
+      if (ki < bk) {
+        bk = ki;
+        best = i;
+      }
+    }
+    for (int i = best; i > pos; --i) schur_swap(n, t, z, i - 1);
+  }
+  lam.resize(p);
+  for (int i = 0; i < p; ++i) lam[i] = t[i + size_t(n) * i];
+  leading_eigvecs(n, p, t, z, anorm, vec);
+  return 0;
+}
 // LU with partial pivoting, solve A X = B in place (B n x m).  Returns 0 or
 // the 1-based index of a zero pivot.
 static int zgesv(int n, std::vector<cd> a, int m, cd *b, int ldb) {
@@ -351,9 +434,9 @@ int harmonic_pencil(int k, const cd *kp, const cd *mp, int ld, int nev, std::vec
                     std::vector<cd> &mu) {
   if (nev > k) return DEX_EGEVP;
   std::vector<cd> lam, vec;
-  std::vector<double> key(k);
   bool ok = false;
-  // Route 1: K = R^H R, C = R^{-H} M R^{-1}, nu = 1/mu, y = R^{-1} z.
+  // Route 1: K = R^H R, C = R^{-H} M R^{-1}, nu = 1/mu, y = R^{-1} z; the
+  // wanted pairs (smallest |mu|) are those of largest |nu|.
   {
     std::vector<cd> r(size_t(k) * k), c(size_t(k) * k);
     for (int j = 0; j < k; ++j)
@@ -374,19 +457,17 @@ int harmonic_pencil(int k, const cd *kp, const cd *mp, int ld, int nev, std::vec
           for (int l = 0; l <= i; ++l) s += std::conj(rinv[l + size_t(k) * i]) * t[l + size_t(k) * j];
           c[i + size_t(k) * j] = s;
         }
-      if (zgeev_right(k, c.data(), k, lam, vec) == 0) {
-        std::vector<cd> y(size_t(k) * k);
-        for (int j = 0; j < k; ++j)
+      auto key = [](const cd &nu) { return std::abs(nu) > 0 ? 1.0 / std::abs(nu) : HUGE_VAL; };
+      if (select_eigen(k, c.data(), k, nev, key, lam, vec) == 0) {
+        std::vector<cd> y(size_t(k) * nev);
+        for (int j = 0; j < nev; ++j)
           for (int i = 0; i < k; ++i) {
             cd s = 0.0;
             for (int l = i; l < k; ++l) s += rinv[i + size_t(k) * l] * vec[l + size_t(k) * j];
             y[i + size_t(k) * j] = s;
           }
         vec.swap(y);
-        for (int i = 0; i < k; ++i) {
-          key[i] = std::abs(lam[i]) > 0 ? 1.0 / std::abs(lam[i]) : HUGE_VAL;
-          lam[i] = std::abs(lam[i]) > 0 ? 1.0 / lam[i] : cd(HUGE_VAL);
-        }
+        for (int j = 0; j < nev; ++j) lam[j] = std::abs(lam[j]) > 0 ? 1.0 / lam[j] : cd(HUGE_VAL);
         ok = true;
       }
     }
@@ -400,24 +481,18 @@ int harmonic_pencil(int k, const cd *kp, const cd *mp, int ld, int nev, std::vec
         c[i + size_t(k) * j] = kp[i + size_t(ld) * j];
       }
     if (zgesv(k, m, k, c.data(), k) != 0) return DEX_EGEVP;
-    if (zgeev_right(k, c.data(), k, lam, vec) != 0) return DEX_EGEVP;
-    for (int i = 0; i < k; ++i) key[i] = std::abs(lam[i]);
+    auto key = [](const cd &x) { return std::isfinite(std::abs(x)) ? std::abs(x) : HUGE_VAL; };
+    if (select_eigen(k, c.data(), k, nev, key, lam, vec) != 0) return DEX_EGEVP;
   }
-  for (int i = 0; i < k; ++i)
-    if (!std::isfinite(key[i])) key[i] = HUGE_VAL;
-  std::vector<int> idx(k);
-  std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key[a] < key[b]; });
   coords.assign(size_t(k) * nev, 0.0);
   mu.resize(nev);
   for (int j = 0; j < nev; ++j) {
-    const int s = idx[j];
     double nrm = 0.0;
-    for (int i = 0; i < k; ++i) nrm += std::norm(vec[i + size_t(k) * s]);
+    for (int i = 0; i < k; ++i) nrm += std::norm(vec[i + size_t(k) * j]);
     nrm = std::sqrt(nrm);
     if (!(nrm > 0) || !std::isfinite(nrm)) return DEX_EGEVP;
-    for (int i = 0; i < k; ++i) coords[i + size_t(k) * j] = vec[i + size_t(k) * s] / nrm;
-    mu[j] = lam[s];
+    for (int i = 0; i < k; ++i) coords[i + size_t(k) * j] = vec[i + size_t(k) * j] / nrm;
+    mu[j] = lam[j];
   }
   return 0;
 }

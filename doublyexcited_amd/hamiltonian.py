@@ -33,6 +33,15 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
+def _dev(t):
+    """Device pointer of a torch tensor, after the work torch has queued on its
+    current stream (which may still be writing the tensor) has finished."""
+    import torch
+
+    torch.cuda.current_stream(t.device).synchronize()
+    return C.c_void_p(t.data_ptr())
+
+
 @dataclasses.dataclass
 class SolveResult:
     e: np.ndarray
@@ -135,19 +144,19 @@ class HeliumGrid:
                       symmetry: int = 0) -> None:
         """HAMMULT on device tensors; symmetry = +1/-1 uses the exchange shortcut."""
         howmany = x.numel() // self.d if howmany is None else howmany
-        check(self._lib.dex_matvec_sym_device(self._ctx, C.c_void_p(x.data_ptr()),
-                                              C.c_void_p(y.data_ptr()), howmany, self.d, int(single),
-                                              int(symmetry)))
+        check(self._lib.dex_matvec_sym_device(self._ctx, _dev(x), _dev(y), howmany, self.d,
+                                              int(single), int(symmetry)))
+        self.sync()
 
     def dots_device(self, bras, kets, nn: int, mm: int) -> np.ndarray:
         out = np.zeros((nn, mm), dtype=np.complex128, order="F")
-        check(self._lib.dex_dots_device(self._ctx, C.c_void_p(bras.data_ptr()), C.c_void_p(kets.data_ptr()),
-                                        self.d, nn, mm, out.ctypes.data_as(_dp)))
+        check(self._lib.dex_dots_device(self._ctx, _dev(bras), _dev(kets), self.d, nn, mm,
+                                        out.ctypes.data_as(_dp)))
         return out
 
     def fill_gaussian_device(self, x, ncols: int, seed: int, stream_id: int = 0, symmetry: int = 1):
-        check(self._lib.dex_fill_gaussian_device(self._ctx, C.c_void_p(x.data_ptr()), ncols, seed,
-                                                 stream_id, int(symmetry)))
+        check(self._lib.dex_fill_gaussian_device(self._ctx, _dev(x), ncols, seed, stream_id,
+                                                 int(symmetry)))
 
     def sync(self):
         check(self._lib.dex_sync(self._ctx))
@@ -164,7 +173,7 @@ class HeliumGrid:
         nev = e.size
         ee = np.ascontiguousarray(e, dtype=np.complex128)
         out = np.zeros(nev)
-        check(self._lib.dex_residuals_device(self._ctx, C.c_void_p(x.data_ptr()), ee.view(np.float64).ctypes.data_as(_dp),
+        check(self._lib.dex_residuals_device(self._ctx, _dev(x), ee.view(np.float64).ctypes.data_as(_dp),
                                              nev, _ptr(out)))
         return out
 
@@ -223,7 +232,7 @@ class HeliumGrid:
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
         o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock)
-        st = self._lib.dex_gplmr_eig_device(self._ctx, C.c_void_p(x.data_ptr()),
+        st = self._lib.dex_gplmr_eig_device(self._ctx, _dev(x),
                                             e.view(np.float64).ctypes.data_as(_dp), self.m, self.m,
                                             float(np.real(sigma)), float(np.imag(sigma)), nev, ns,
                                             maxit, tol, inner_iter, outmodulus, C.byref(o), C.byref(r))
