@@ -677,66 +677,75 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double2 *__restri
   if (lane == 0) out[e] = make_double2(sx, sy);
 }
 
-constexpr int kCombRows = 64;
+constexpr int kCombRows = 32;
 constexpr int kCombThreads = 256;
-constexpr int kCombMaxIn = 96;
-constexpr int kCombMaxOut = 48;
-constexpr int kCombJ = 8;  // outputs per thread
+constexpr int kCombMaxIn = 64;
+constexpr int kCombGroups = 8;  // output groups per workgroup (kCombThreads / kCombRows)
 
-// out_j = sum_i in_i C(i, j) (+ beta_j out_j).  A workgroup takes 64 rows;
-// thread t owns row t % 64 and the 8 outputs starting at 8 (t / 64) (then
-// +32, ...), so C(i, j) is uniform across a wave.  The input tile is stored
-// column by column in LDS (conflict-free reads by consecutive rows).
+// out_j = sum_i in_i C(i, j) (+ beta_j out_j), row by row.  A workgroup stages
+// C (nin x nout, padded to kCombGroups * JPT columns) once in LDS, then walks
+// tiles of kCombRows rows: the tile of every input column goes to LDS, and
+// thread t forms, for row t % kCombRows, the JPT outputs of group
+// t / kCombRows.  All loads of a tile finish before any of its outputs is
+// written, so an output may alias an input.
+template <int JPT>
 __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restrict__ descs, int nin,
                                                           int nout, const double2 *__restrict__ cm,
                                                           const double2 *__restrict__ beta, int64_t d) {
-  __shared__ double2 tin[kCombMaxIn * kCombRows];
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NP = kCombGroups * JPT;
+  double2 *sc = reinterpret_cast<double2 *>(smem);  // [nin][NP]
+  double2 *tin = sc + size_t(nin) * NP;              // [nin][kCombRows]
   const int tid = threadIdx.x;
   const int row = tid % kCombRows;
-  const int wj = __builtin_amdgcn_readfirstlane(tid / kCombRows);
+  const int grp = tid / kCombRows;
+  for (int e = tid; e < nin * NP; e += kCombThreads) {
+    const int i = e / NP, j = e % NP;
+    sc[e] = j < nout ? cm[i + size_t(nin) * j] : make_double2(0.0, 0.0);
+  }
   const int64_t ntile = (d + kCombRows - 1) / kCombRows;
   for (int64_t tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
     const int64_t rb = tb * kCombRows;
     const int nr = int(d - rb < kCombRows ? d - rb : kCombRows);
+    __syncthreads();
     for (int e = tid; e < kCombRows * nin; e += kCombThreads) {
       const int r = e % kCombRows, c = e / kCombRows;
       if (r < nr) tin[c * kCombRows + r] = col_value(descs[c], rb + r);
     }
     __syncthreads();
     if (row < nr) {
-      for (int j0 = wj * kCombJ; j0 < nout; j0 += (kCombThreads / kCombRows) * kCombJ) {
-        double sx[kCombJ], sy[kCombJ];
+      double ax[JPT], ay[JPT];
 #pragma unroll
-        for (int q = 0; q < kCombJ; ++q) sx[q] = sy[q] = 0.0;
-        for (int i = 0; i < nin; ++i) {
-          const double2 a = tin[i * kCombRows + row];
+      for (int q = 0; q < JPT; ++q) ax[q] = ay[q] = 0.0;
+      const double2 *cg = sc + grp * JPT;
+#pragma unroll 4
+      for (int i = 0; i < nin; ++i) {
+        const double2 a = tin[i * kCombRows + row];
 #pragma unroll
-          for (int q = 0; q < kCombJ; ++q) {
-            if (j0 + q < nout) {
-              const double2 c = cm[i + size_t(nin) * (j0 + q)];
-              sx[q] += a.x * c.x - a.y * c.y;
-              sy[q] += a.x * c.y + a.y * c.x;
-            }
-          }
+        for (int q = 0; q < JPT; ++q) {
+          const double2 c = cg[i * NP + q];
+          ax[q] = fma(a.x, c.x, ax[q]);
+          ax[q] = fma(-a.y, c.y, ax[q]);
+          ay[q] = fma(a.x, c.y, ay[q]);
+          ay[q] = fma(a.y, c.x, ay[q]);
         }
+      }
 #pragma unroll
-        for (int q = 0; q < kCombJ; ++q) {
-          const int j = j0 + q;
-          if (j < nout) {
-            double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + j].a)) + rb + row;
-            double ox = sx[q], oy = sy[q];
-            if (beta != nullptr) {
-              const double2 bj = beta[j];
-              const double2 ov = *o;
-              ox += bj.x * ov.x - bj.y * ov.y;
-              oy += bj.x * ov.y + bj.y * ov.x;
-            }
-            *o = make_double2(ox, oy);
+      for (int q = 0; q < JPT; ++q) {
+        const int j = grp * JPT + q;
+        if (j < nout) {
+          double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + j].a)) + rb + row;
+          double ox = ax[q], oy = ay[q];
+          if (beta != nullptr) {
+            const double2 bj = beta[j];
+            const double2 ov = *o;
+            ox += bj.x * ov.x - bj.y * ov.y;
+            oy += bj.x * ov.y + bj.y * ov.x;
           }
+          *o = make_double2(ox, oy);
         }
       }
     }
-    __syncthreads();
   }
 }
 
@@ -917,14 +926,37 @@ hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t 
   return hipGetLastError();
 }
 
-hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
+namespace {
+template <int JPT>
+hipError_t combine_launch(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
                           int64_t d, hipStream_t s) {
-  if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombMaxOut) return hipErrorInvalidValue;
+  const size_t lds = (size_t(nin) * kCombGroups * JPT + size_t(kCombRows) * nin) * sizeof(double2);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_combine<JPT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
   const int64_t ntile = (d + kCombRows - 1) / kCombRows;
-  const unsigned grid = unsigned(ntile < 4096 ? ntile : 4096);
-  hipLaunchKernelGGL(k_combine, dim3(grid), dim3(kCombThreads), 0, s, descs, nin, nout,
+  const unsigned grid = unsigned(ntile < 2048 ? ntile : 2048);
+  hipLaunchKernelGGL((k_combine<JPT>), dim3(grid), dim3(kCombThreads), lds, s, descs, nin, nout,
                      reinterpret_cast<const double2 *>(c), reinterpret_cast<const double2 *>(beta), d);
   return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
+                          int64_t d, hipStream_t s) {
+  if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombGroups * 6) return hipErrorInvalidValue;
+  switch ((nout + kCombGroups - 1) / kCombGroups) {
+    case 1: return combine_launch<1>(descs, nin, nout, c, beta, d, s);
+    case 2: return combine_launch<2>(descs, nin, nout, c, beta, d, s);
+    case 3: return combine_launch<3>(descs, nin, nout, c, beta, d, s);
+    case 4: return combine_launch<4>(descs, nin, nout, c, beta, d, s);
+    case 5: return combine_launch<5>(descs, nin, nout, c, beta, d, s);
+    default: return combine_launch<6>(descs, nin, nout, c, beta, d, s);
+  }
 }
 
 size_t coldots_partial_elems(int k) { return size_t(kDotBlocks) * k * 4; }
