@@ -31,7 +31,7 @@ EXPORTS = [
     "dex_get_kematrix", "dex_get_tinv", "dex_matvec", "dex_matvec_device", "dex_matvec_sym_device",
     "dex_dots",
     "dex_dots_device", "dex_solve_opts_default", "dex_gplmr_eig", "dex_gplmr_eig_device",
-    "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint",
+    "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint", "dex_precblock_device",
     "dex_status_string", "dex_last_error", "dex_set_kernel_timing", "dex_get_kernel_timing",
     "dex_host_kematrix", "dex_host_getinverse", "dex_host_solve_gevp",
 ]
@@ -60,7 +60,8 @@ class DexResult(C.Structure):
 class DexSolveOpts(C.Structure):
     _fields_ = [("strict_cholesky", C.c_int32), ("verbose", C.c_int32),
                 ("refresh_every", C.c_int32), ("fp64_restarts", C.c_int32),
-                ("fp32_tol", C.c_double), ("symmetry", C.c_int32), ("lock", C.c_int32)]
+                ("fp32_tol", C.c_double), ("symmetry", C.c_int32), ("lock", C.c_int32),
+                ("inner_block", C.c_int32)]
 
 
 class DexError(RuntimeError):
@@ -103,7 +104,9 @@ def load() -> C.CDLL:
         "dex_fill_gaussian_device": (C.c_int, [P, P, C.c_int32, C.c_uint64, C.c_uint32, C.c_int32]),
         "dex_sync": (C.c_int, [P]),
         "dex_residuals_device": (C.c_int, [P, P, dp, C.c_int32, dp]),
-        "dex_gplmr_footprint": (C.c_int64, [P, C.c_int32, C.c_int32]),
+        "dex_gplmr_footprint": (C.c_int64, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+        "dex_precblock_device": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                           C.c_double, C.c_double, C.c_int32, C.c_int32]),
         "dex_status_string": (C.c_char_p, [C.c_int]),
         "dex_last_error": (C.c_char_p, []),
         "dex_set_kernel_timing": (C.c_int, [P, C.c_int32]),

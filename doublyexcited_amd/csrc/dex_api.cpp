@@ -533,7 +533,7 @@ struct Solver {
   int sym;      // 0, or +-1 for exchange-(anti)symmetric blocks
   int fallbacks = 0;
   int64_t matvecs = 0;
-  double t_matvec = 0, t_gram = 0, t_update = 0, t_small = 0;
+  double t_matvec = 0, t_gram = 0, t_update = 0, t_small = 0, t_prec = 0;
   std::vector<Dbl2 *> pool;
 
   int matblock(const Block &in, const Block &out) {
@@ -656,21 +656,126 @@ struct Solver {
     t_gram += now_s() - t0;
     t0 = now_s();
     const int kq = q.size(), kb = b.size();
-    // C = [I; -G]
-    std::vector<cd> cm(size_t(kb + kq) * kb, 0.0);
-    for (int j = 0; j < kb; ++j) {
-      cm[j + size_t(kb + kq) * j] = 1.0;
-      for (int i = 0; i < kq; ++i) cm[kb + i + size_t(kb + kq) * j] = -g[i + size_t(kq) * j];
-    }
-    std::vector<ColDesc> in = descs_of(b);
-    for (Dbl2 *p : q.cols) in.push_back(plain(p));
-    if (int st = combine(c, in, b.cols, cm)) return st;
-    if (ab) {
-      std::vector<ColDesc> ain = descs_of(*ab);
-      for (Dbl2 *p : aq->cols) ain.push_back(plain(p));
-      if (int st = combine(c, ain, ab->cols, cm)) return st;
+    // b <- [b q] [I; -G]: q in chunks that fit one launch, the first chunk
+    // carrying b itself (in place), later ones accumulating with beta = 1.
+    std::vector<cd> one(kb, 1.0);
+    for (int q0 = 0; q0 < kq;) {
+      const bool first = q0 == 0;
+      const int nq = std::min(first ? 64 - kb : 64, kq - q0);
+      const int nin = (first ? kb : 0) + nq;
+      const int off = first ? kb : 0;
+      std::vector<cd> cm(size_t(nin) * kb, 0.0);
+      for (int j = 0; j < kb; ++j) {
+        if (first) cm[j + size_t(nin) * j] = 1.0;
+        for (int i = 0; i < nq; ++i) cm[off + i + size_t(nin) * j] = -g[q0 + i + size_t(kq) * j];
+      }
+      std::vector<ColDesc> in;
+      if (first) in = descs_of(b);
+      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i]));
+      if (int st = combine(c, in, b.cols, cm, first ? nullptr : &one)) return st;
+      if (ab) {
+        std::vector<ColDesc> ain;
+        if (first) ain = descs_of(*ab);
+        for (int i = 0; i < nq; ++i) ain.push_back(plain(aq->cols[q0 + i]));
+        if (int st = combine(c, ain, ab->cols, cm, first ? nullptr : &one)) return st;
+      }
+      q0 += nq;
     }
     t_update += now_s() - t0;
+    return 0;
+  }
+
+  // PRECBLOCK_BASIC -> basicblocklansolve (gplmr.F90:674-770;
+  // harmonic_ritz_small.F90:258-324): b <- argmin over x in K_m(H, b) of
+  // ||(H - sigma) x - b||, the fixed-order unrestarted block minimal-residual
+  // approximation to (H - sigma)^{-1} b, for every column of the block (the
+  // reference's allhmultsub multiplies only the first, gplmr.F90:714).
+  //
+  // Same Krylov space and the same least-squares solution as the reference,
+  // with one matvec per basis column instead of two: the reference forms
+  // V_j = H V_{j-1} by a matvec and then B_j = (H - sigma) V_j by another
+  // (harmonic_ritz_small.F90:122-127); here V_j = B_{j-1} + sigma V_{j-1},
+  // since every transform applied to B is applied to V as well.  B is made
+  // orthonormal block by block (two projections against the earlier blocks,
+  // as myhgramschmidt_many's two passes, then cholQR), so the solution is
+  // x = V (B^H b).
+  int inner_solve(Block &b, int m, cd sigma) {
+    const int k = b.size();
+    if (k == 0 || m < 1) return 0;
+    const double t0 = now_s();
+    Block vall, ball;
+    Block vj, bj;
+    if (int st = grow(vj, k)) return st;
+    if (int st = grow(bj, k)) return st;
+    {
+      std::vector<cd> id(size_t(k) * k, 0.0);
+      for (int j = 0; j < k; ++j) id[j + size_t(k) * j] = 1.0;
+      if (int st = combine(c, descs_of(b), vj.cols, id)) return st;  // V_1 = b
+    }
+    for (int it = 0; it < m; ++it) {
+      if (it > 0) {
+        // V_j = H V_{j-1} = B_{j-1} + sigma V_{j-1}
+        const int kp = vj.size();
+        Block vn, bn;
+        if (int st = grow(vn, kp)) return st;
+        if (int st = grow(bn, kp)) return st;
+        std::vector<cd> cm(size_t(2 * kp) * kp, 0.0);
+        for (int j = 0; j < kp; ++j) {
+          cm[j + size_t(2 * kp) * j] = 1.0;
+          cm[kp + j + size_t(2 * kp) * j] = sigma;
+        }
+        std::vector<ColDesc> in = descs_of(bj);
+        for (Dbl2 *p : vj.cols) in.push_back(plain(p));
+        if (int st = combine(c, in, vn.cols, cm)) return st;
+        vj = vn;
+        bj = bn;
+      }
+      // B_j = (H - sigma) V_j
+      if (int st = matblock(vj, bj)) return st;
+      {
+        const int kj = vj.size();
+        std::vector<cd> cm(size_t(kj) * kj, 0.0);
+        for (int j = 0; j < kj; ++j) cm[j + size_t(kj) * j] = 1.0;
+        std::vector<ColDesc> in;
+        for (int j = 0; j < kj; ++j) in.push_back(shifted(bj.cols[j], vj.cols[j], sigma));
+        if (int st = combine(c, in, bj.cols, cm)) return st;
+      }
+      for (int pass = 0; pass < 2 && ball.size() > 0; ++pass)
+        if (int st = project_out(ball, &vall, bj, &vj)) return st;
+      const int kk = orth(bj, &vj, true);
+      if (kk < 0) return -kk;
+      if (bj.size() == 0) break;  // the Krylov space is exhausted
+      ball.cols.insert(ball.cols.end(), bj.cols.begin(), bj.cols.end());
+      vall.cols.insert(vall.cols.end(), vj.cols.begin(), vj.cols.end());
+    }
+    // x = V (B^H b), written over b
+    std::vector<cd> y;
+    double tg = now_s();
+    if (int st = gram(c, descs_of(ball), descs_of(b), y)) return st;
+    t_gram += now_s() - tg;
+    tg = now_s();
+    if (int st = combine_chunked(vall, y, b)) return st;
+    t_update += now_s() - tg;
+    for (Dbl2 *p : vall.cols) pool.push_back(p);
+    for (Dbl2 *p : ball.cols) pool.push_back(p);
+    t_prec += now_s() - t0;
+    return 0;
+  }
+
+  // out = in C for an input block of any width (64 columns per launch,
+  // accumulated with beta = 1); `out` must not alias `in`.
+  int combine_chunked(const Block &in, const std::vector<cd> &cm, Block &out) {
+    const int nin = in.size(), nout = out.size();
+    for (int i0 = 0; i0 < nin; i0 += 64) {
+      const int bi = std::min(64, nin - i0);
+      std::vector<ColDesc> ds;
+      for (int i = 0; i < bi; ++i) ds.push_back(plain(in.cols[i0 + i]));
+      std::vector<cd> sub(size_t(bi) * nout);
+      for (int j = 0; j < nout; ++j)
+        for (int i = 0; i < bi; ++i) sub[i + size_t(bi) * j] = cm[i0 + i + size_t(nin) * j];
+      std::vector<cd> one(nout, 1.0);
+      if (int st = combine(c, ds, out.cols, sub, i0 > 0 ? &one : nullptr)) return st;
+    }
     return 0;
   }
 
@@ -731,6 +836,24 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     }
     if (int st = combine(c, descs_of(X), X.cols, dm)) return st;
   }
+  auto precblock = [&](Block &b) -> int {
+    if (inner == 0 || b.size() == 0) return 0;
+    if (inner == 1) {
+      Block hb;
+      if (int st = S.grow(hb, b.size())) return st;
+      if (int st = S.matblock(b, hb)) return st;
+      for (Dbl2 *p : b.cols) S.pool.push_back(p);
+      b = hb;
+      return 0;
+    }
+    const int grp = o.inner_block > 0 ? o.inner_block : b.size();
+    for (int j0 = 0; j0 < b.size(); j0 += grp) {
+      Block sub;
+      sub.cols.assign(b.cols.begin() + j0, b.cols.begin() + std::min(b.size(), j0 + grp));
+      if (int st = S.inner_solve(sub, inner, sigma)) return st;
+    }
+    return 0;
+  };
   if (int st = S.matblock(X, AX)) return st;  // :68
   std::vector<cd> e(nev);
   std::vector<double> resn(nev);
@@ -795,14 +918,9 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     // CHOLQR(X); AX <- AX R^{-1}  (:88-90)
     if (int st = S.orth(X, &AX, false)) return -st;
     if (X.size() != nev) return fail(DEX_ECHOLESKY, "X lost rank");
-    // PRECBLOCK with GM_MAX_INNER_ITER = 1: W <- H W (:92-99, :491-493)
-    if (inner == 1) {
-      Block HW;
-      if (int st = S.grow(HW, W.size())) return st;
-      if (int st = S.matblock(W, HW)) return st;
-      for (Dbl2 *p : W.cols) S.pool.push_back(p);
-      W = HW;
-    }
+    // PRECBLOCK (:92-99): GM_MAX_INNER_ITER = 1 gives W <- H W (:491-493),
+    // larger values the inner minimal-residual solve
+    if (int st = precblock(W)) return st;
     // W <- W - X (X^H W); CHOLQR(W)  (:101-105)
     if (int st = S.project_out(X, nullptr, W, nullptr)) return st;
     if (int st = S.orth(W, nullptr, true)) return -st;
@@ -830,13 +948,7 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
         for (Dbl2 *p : prev.cols) in.push_back(plain(p));
         if (int st = combine(c, in, sl.cols, cm)) return st;
       }
-      if (inner == 1 && sl.size() > 0) {
-        Block hs;
-        if (int st = S.grow(hs, sl.size())) return st;
-        if (int st = S.matblock(sl, hs)) return st;
-        for (Dbl2 *p : sl.cols) S.pool.push_back(p);
-        sl = hs;
-      }
+      if (int st = precblock(sl)) return st;  // :118-126, :148-156
       if (int st = S.project_out(X, nullptr, sl, nullptr)) return st;
       if (int st = S.project_out(W, nullptr, sl, nullptr)) return st;
       if (int st = S.project_out(Sblk, nullptr, sl, nullptr)) return st;
@@ -942,6 +1054,7 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     res->t_gram += S.t_gram;
     res->t_update += S.t_update;
     res->t_smalleig += S.t_small;
+    res->t_prec += S.t_prec;
     res->fallbacks += S.fallbacks;
   }
   return maxres <= tol ? DEX_OK : DEX_ENOTCONVERGED;
@@ -949,12 +1062,16 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
 
 }  // namespace
 
-extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns) {
+extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns, int32_t inner_iter,
+                                       int32_t inner_block) {
   if (!c) return -1;
-  // The caller's block X (NEV), the solver's (6 + 2 NS) NEV work vectors and
-  // the matvec intermediate U.
+  // The caller's block X (NEV), the solver's (6 + 2 NS) NEV work vectors, the
+  // matvec intermediate U, and for an inner solve of order m on groups of g
+  // columns its 2 m g basis vectors (V and B) plus 2 g for the next Krylov block.
   const int64_t vec = c->d * 16;
-  return int64_t(7 + 2 * ns) * nev * vec + vec;
+  const int64_t grp = inner_block > 0 && inner_block < nev ? inner_block : nev;
+  const int64_t inner = inner_iter >= 2 ? int64_t(2 * inner_iter + 2) * grp : (inner_iter == 1 ? nev : 0);
+  return (int64_t(7 + 2 * ns) * nev + inner) * vec + vec;
 }
 
 extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,
@@ -964,7 +1081,7 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   if (!c || !x_dev || !e) return fail(DEX_EINVAL, "null argument");
   if (int64_t(m) * mb != c->d) return fail(DEX_ESIZE, "SIZE ERROR: M*MB != NN^6");
   if (nev < 1 || ns < 1 || (3 + ns) * nev > 64) return fail(DEX_EINVAL, "need NEV >= 1, NS >= 1, (3+NS)*NEV <= 64");
-  if (inner_iter != 0 && inner_iter != 1) return fail(DEX_EUNSUPPORTED, "GM_MAX_INNER_ITER must be 0 or 1");
+  if (inner_iter < 0) return fail(DEX_EINVAL, "GM_MAX_INNER_ITER must be >= 0");
   if (opts && (opts->symmetry < -1 || opts->symmetry > 1)) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
   if (opts && opts->symmetry != 0 && !c->fast64 && !(c->p.precision == DEX_FP32 && c->fast32))
     return fail(DEX_EUNSUPPORTED, "exchange-symmetric matvec needs the LDS kernels for this grid");
@@ -1006,6 +1123,7 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
         res->t_gram += sub.t_gram;
         res->t_update += sub.t_update;
         res->t_smalleig += sub.t_smalleig;
+        res->t_prec += sub.t_prec;
         res->fallbacks += sub.fallbacks;
         res->max_residual = sub.max_residual;
         res->converged = sub.converged;
@@ -1019,6 +1137,40 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
     e[2 * j] = ev[j].real();
     e[2 * j + 1] = ev[j].imag();
   }
+  return st;
+}
+
+extern "C" int dex_precblock_device(dex_ctx *c, const void *x_dev, void *px_dev, int32_t m, int32_t mb,
+                                    int32_t nev, double sigma_re, double sigma_im, int32_t inner_iter,
+                                    int32_t symmetry) {
+  if (!c || !x_dev || !px_dev) return fail(DEX_EINVAL, "null argument");
+  if (int64_t(m) * mb != c->d) return fail(DEX_ESIZE, "SIZE ERROR: M*MB != NN^6");
+  if (nev < 1 || nev > 48) return fail(DEX_EINVAL, "need 1 <= NEV <= 48");
+  if (inner_iter < 1) return fail(DEX_EINVAL, "PRECBLOCK needs GM_MAX_INNER_ITER >= 1");  // gplmr.F90:487-489
+  if (symmetry < -1 || symmetry > 1) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
+  if (symmetry != 0 && !c->fast64) return fail(DEX_EUNSUPPORTED, "exchange-symmetric matvec needs the LDS kernels for this grid");
+  if (x_dev == px_dev) return fail(DEX_EINVAL, "X and PX must not alias");
+  DEX_HIP(hipSetDevice(c->p.device));
+  Solver S{c, nev, 1, false, false, symmetry};
+  std::vector<Dbl2 *> store;
+  S.store = &store;
+  Block in, out;
+  for (int j = 0; j < nev; ++j) {
+    in.cols.push_back(const_cast<Dbl2 *>(static_cast<const Dbl2 *>(x_dev)) + size_t(j) * c->d);
+    out.cols.push_back(static_cast<Dbl2 *>(px_dev) + size_t(j) * c->d);
+  }
+  int st;
+  if (inner_iter == 1) {
+    st = S.matblock(in, out);
+  } else {
+    st = 0;
+    const size_t bytes = size_t(c->d) * nev * sizeof(Dbl2);
+    if (hipMemcpyAsync(px_dev, x_dev, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      st = fail(DEX_EHIP, "device copy failed");
+    if (!st) st = S.inner_solve(out, inner_iter, cd(sigma_re, sigma_im));
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = fail(DEX_EHIP, "stream synchronise failed");
+  for (Dbl2 *p : store) (void)hipFree(p);
   return st;
 }
 

@@ -52,6 +52,7 @@ class SolveResult:
     max_residual: float
     t_total: float
     t_matvec: float
+    t_prec: float
     t_gram: float
     t_update: float
     t_smalleig: float
@@ -177,17 +178,25 @@ class HeliumGrid:
                                              nev, _ptr(out)))
         return out
 
-    def footprint(self, nev: int, ns: int = 1) -> int:
-        return int(self._lib.dex_gplmr_footprint(self._ctx, nev, ns))
+    def precblock_device(self, x, px, nev: int, sigma: complex, inner_iter: int, symmetry: int = 0) -> None:
+        """PRECBLOCK (gplmr.F90:466-502) on device blocks: px = H x for
+        inner_iter = 1, else the order-inner_iter block minimal-residual
+        approximation to (H - sigma)^{-1} x (PRECBLOCK_BASIC)."""
+        check(self._lib.dex_precblock_device(self._ctx, _dev(x), _dev(px), self.m, self.m, nev,
+                                             float(sigma.real), float(sigma.imag), inner_iter, symmetry))
+
+    def footprint(self, nev: int, ns: int = 1, inner_iter: int = 0, inner_block: int = 0) -> int:
+        return int(self._lib.dex_gplmr_footprint(self._ctx, nev, ns, inner_iter, inner_block))
 
     # -- eigensolver --------------------------------------------------------------------
     @staticmethod
     def _opts(strict: bool, verbose: bool, refresh_every: int, fp64_restarts: int, fp32_tol: float,
-              symmetry: int = 0, lock: bool = False):
+              symmetry: int = 0, lock: bool = False, inner_block: int = 0):
         o = _lib.DexSolveOpts()
         _lib.load().dex_solve_opts_default(C.byref(o))
         o.symmetry = int(symmetry)
         o.lock = int(lock)
+        o.inner_block = int(inner_block)
         o.strict_cholesky = int(strict)
         o.verbose = int(verbose)
         o.refresh_every = refresh_every
@@ -198,7 +207,7 @@ class HeliumGrid:
     def _result(self, st, r, e):
         return SolveResult(e=e, residuals=None, iterations=r.iterations, converged=bool(r.converged),
                            matvecs=r.matvecs, max_residual=r.max_residual, t_total=r.t_total,
-                           t_matvec=r.t_matvec, t_gram=r.t_gram, t_update=r.t_update,
+                           t_matvec=r.t_matvec, t_prec=r.t_prec, t_gram=r.t_gram, t_update=r.t_update,
                            t_smalleig=r.t_smalleig, fallbacks=r.fallbacks,
                            fp64_restarts_used=r.fp64_restarts_used, status=st)
 
@@ -206,7 +215,7 @@ class HeliumGrid:
                   tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5, strict: bool = False,
                   verbose: bool = False, refresh_every: int = 0, fp64_restarts: int = 2,
                   fp32_tol: float = 1e-5, m: int | None = None, mb: int | None = None,
-                  symmetry: int = 0, lock: bool = False) -> SolveResult:
+                  symmetry: int = 0, lock: bool = False, inner_block: int = 0) -> SolveResult:
         """GPLMR_EIG on a host block ``x`` (D*NEV complex128, vector-major), updated in place."""
         if x.dtype != np.complex128 or not (x.flags.f_contiguous or x.flags.c_contiguous):
             raise TypeError("x must be a contiguous complex128 array")
@@ -216,7 +225,7 @@ class HeliumGrid:
             raise DexError(_lib.DEX_ESIZE, "x.size != M*MB*NEV")
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
-        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock)
+        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock, inner_block)
         st = self._lib.dex_gplmr_eig(self._ctx, _ptr(x), e.view(np.float64).ctypes.data_as(_dp), m, mb,
                                      float(np.real(sigma)), float(np.imag(sigma)), nev, ns, maxit, tol,
                                      inner_iter, outmodulus, C.byref(o), C.byref(r))
@@ -227,11 +236,11 @@ class HeliumGrid:
                          tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5,
                          strict: bool = False, verbose: bool = False, refresh_every: int = 0,
                          fp64_restarts: int = 2, fp32_tol: float = 1e-5,
-                         symmetry: int = 0, lock: bool = False) -> SolveResult:
+                         symmetry: int = 0, lock: bool = False, inner_block: int = 0) -> SolveResult:
         """GPLMR_EIG with the block resident on the device (torch complex128 tensor)."""
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
-        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock)
+        o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock, inner_block)
         st = self._lib.dex_gplmr_eig_device(self._ctx, _dev(x),
                                             e.view(np.float64).ctypes.data_as(_dp), self.m, self.m,
                                             float(np.real(sigma)), float(np.imag(sigma)), nev, ns,

@@ -59,11 +59,13 @@ def snap_nn(nn: int) -> int:
     return (nn - 1) // 2
 
 
-def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1) -> int:
-    """Device bytes a solve holds: the block X, (6 + 2 NS) NEV work vectors and
-    the matvec intermediate (the same for both precisions: the basis is complex128)."""
+def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1, inner_iter: int = 0) -> int:
+    """Device bytes a solve holds: the block X, (6 + 2 NS) NEV work vectors, the
+    matvec intermediate (the same for both precisions: the basis is complex128)
+    and, for an inner solve of order m >= 2, its (2 m + 2) NEV basis vectors."""
     d = (2 * n + 1) ** 6
-    return (7 + 2 * ns) * nev * 16 * d + 16 * d
+    inner = (2 * inner_iter + 2) * nev if inner_iter >= 2 else (nev if inner_iter == 1 else 0)
+    return ((7 + 2 * ns) * nev + inner) * 16 * d + 16 * d
 
 
 def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),

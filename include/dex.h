@@ -134,6 +134,10 @@ typedef struct dex_solve_opts {
   int32_t lock;            /* 1: build W, S, P only for columns whose residual is above
                               TOL (all columns stay in X and the projection); 0 (default):
                               every column, as the reference */
+  int32_t inner_block;     /* inner solve (inner_iter >= 2): >0 solves column groups of
+                              this many at a time, each in its own block Krylov space
+                              (2 m inner_block basis vectors instead of 2 m NEV); 0
+                              (default): the whole block at once, as the reference */
 } dex_solve_opts;
 
 void dex_solve_opts_default(dex_solve_opts *o);
@@ -142,8 +146,11 @@ void dex_solve_opts_default(dex_solve_opts *o);
  *           PRECONDITIONER,OUTMODULUS), gplmr.F90:9.
  * X (M*MB*NEV complex, host): start block in, eigenvectors out.  E (NEV
  * complex, host): Rayleigh quotients out.  M*MB must equal D (one process
- * holds the whole vector).  inner_iter is GM_MAX_INNER_ITER: 0 (no inner
- * solve) or 1 (W <- H W), gplmr.F90:466-502.  res/opts may be NULL. */
+ * holds the whole vector).  inner_iter is GM_MAX_INNER_ITER (PRECBLOCK,
+ * gplmr.F90:466-502): 0 no inner solve, 1 W <- H W, m >= 2 the in-house
+ * order-m block minimal-residual solve of (H - sigma) x = b
+ * (PRECBLOCK_BASIC, gplmr.F90:674-770) applied to every column of the
+ * block.  res/opts may be NULL. */
 int dex_gplmr_eig(dex_ctx *ctx, double *x, double *e, int32_t m, int32_t mb,
                   double sigma_re, double sigma_im, int32_t nev, int32_t ns,
                   int32_t maxit, double tol, int32_t inner_iter, int32_t outmodulus,
@@ -165,9 +172,23 @@ int dex_sync(dex_ctx *ctx);
 int dex_residuals_device(dex_ctx *ctx, const void *x_dev, const double *e, int32_t nev,
                          double *res_out);
 
-/* Bytes of device memory a dex_gplmr_eig_device solve of (nev, ns) holds on this
- * context, the caller's block X included. */
-int64_t dex_gplmr_footprint(const dex_ctx *ctx, int32_t nev, int32_t ns);
+/* PRECBLOCK(HAM,PARAMS,MPIDATA,GMSOLVER,X,PX,SIGMA,M,MB,NEV,MAX_INNER_ITER,...),
+ * gplmr.F90:466-502, on device blocks of NEV columns (X is read, PX written;
+ * they must not alias).  inner_iter = 1: PX = H X (MATBLOCK); inner_iter =
+ * m >= 2: PX = argmin over the block Krylov space K_m(H, X) of
+ * ||(H - sigma) PX - X|| (PRECBLOCK_BASIC -> basicblocklansolve,
+ * gplmr.F90:674-770, harmonic_ritz_small.F90:258-324), every column
+ * multiplied.  inner_iter = 0 is an error, as in the reference (:487-489).
+ * symmetry: 0, or +-1 for exchange-(anti)symmetric X. */
+int dex_precblock_device(dex_ctx *ctx, const void *x_dev, void *px_dev, int32_t m, int32_t mb,
+                         int32_t nev, double sigma_re, double sigma_im, int32_t inner_iter,
+                         int32_t symmetry);
+
+/* Bytes of device memory a dex_gplmr_eig_device solve of (nev, ns, inner_iter,
+ * inner_block as in dex_solve_opts) holds on this context, the caller's block X
+ * included. */
+int64_t dex_gplmr_footprint(const dex_ctx *ctx, int32_t nev, int32_t ns, int32_t inner_iter,
+                            int32_t inner_block);
 
 /* Host-only set-up and small-matrix routines (no device use), exported so the
  * host arithmetic can be checked on machines without a GPU:

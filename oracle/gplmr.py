@@ -126,6 +126,68 @@ def solve_gevp(kp, mp, k, nev):
     return vr[:, idx], (alpha / beta)[idx]
 
 
+def _herm_sqrt_pair(g):
+    """hermsqrtmatsmooth with tol = 0 (hr_eigen.f90:214-291): G^{-1/2} and G^{1/2}
+    from the SVD of the Hermitian PSD G; zero singular values stay zero."""
+    u, s, vh = np.linalg.svd(g)
+    v = vh.conj().T
+    sinv = np.where(s != 0.0, 1.0 / np.where(s != 0.0, s, 1.0), 0.0)
+    ginvhalf = (v * np.sqrt(sinv)[None, :]) @ u.conj().T
+    ghalf = (v * np.sqrt(s)[None, :]) @ u.conj().T
+    return ginvhalf, ghalf
+
+
+def basic_block_lan_solve(amul, b, sigma, order):
+    """PRECBLOCK_BASIC -> basicblocklansolve (gplmr.F90:674-770;
+    harmonic_ritz_small.F90:258-324): a fixed-order, unrestarted block
+    minimal-residual approximation to (H - sigma)^{-1} b.
+
+    Followed step by step, with every column of the block multiplied (the
+    reference's ``allhmultsub`` multiplies only the first column,
+    gplmr.F90:704-716, which is the NEV > 1 defect it is fixed here):
+
+    * ``harm_orthog_oneblock`` (harmonic_ritz_small.F90:158-245): B0 = (H - sigma) b,
+      G = B0^H B0, V1 = b G^{-1/2} so that (H - sigma) V1 is orthonormal, and
+      ``lantrans`` = G^{1/2};
+    * ``fast_harmritz_setup`` (:100-155): blocks V_k = H V_{k-1}, B_k = (H - sigma) V_k,
+      each column Gram-Schmidt'ed twice (``myhgramschmidt_many``, :28-81, with
+      factors (1, 0): the inner product is that of the B part) against every
+      earlier column, the same combination applied to V;
+    * the solve (:309-318): y = (B^H V) [G^{1/2}; 0], x = V y.
+
+    Returns x with the shape of b.
+    """
+    if order <= 0:
+        raise GplmrError("ORDERERROR")  # harmonic_ritz_small.F90:284-286
+    k = b.shape[1]
+    hb = amul(b) - sigma * b
+    ginvhalf, ghalf = _herm_sqrt_pair(crossmult(hb, hb))
+    v1 = b @ ginvhalf.conj().T  # ZGEMM('N','C') (:204-205)
+    dim = b.shape[0]
+    vall = np.empty((dim, order * k), np.complex128)
+    ball = np.empty((dim, order * k), np.complex128)
+    for io in range(order):
+        v = v1.copy() if io == 0 else amul(vall[:, (io - 1) * k:io * k])
+        bb = amul(v) - sigma * v
+        for i in range(k):
+            col = io * k + i
+            vi, bi = v[:, i], bb[:, i]
+            for _ in range(2):
+                if col:
+                    c = ball[:, :col].conj().T @ bi
+                    bi = bi - ball[:, :col] @ c
+                    vi = vi - vall[:, :col] @ c
+                nrm = np.sqrt(np.vdot(bi, bi).real)
+                bi = bi / nrm
+                vi = vi / nrm
+            ball[:, col] = bi
+            vall[:, col] = vi
+    lanham = ball.conj().T @ vall
+    rhs = np.zeros((order * k, k), np.complex128)
+    rhs[:k] = ghalf
+    return vall @ (lanham @ rhs)
+
+
 @dataclasses.dataclass
 class GplmrResult:
     x: np.ndarray
@@ -152,8 +214,8 @@ def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: c
     """
     if ns < 1:
         raise GplmrError("NS must be >= 1")  # R is (NS*NEV, NEV), gplmr.F90:36
-    if inner_iter not in (0, 1):
-        raise GplmrError("only GM_MAX_INNER_ITER 0 or 1 is restated")
+    if inner_iter < 0:
+        raise GplmrError("GM_MAX_INNER_ITER must be >= 0")
     x = np.array(x0, dtype=np.complex128, copy=True)
     dim = x.shape[0]
     ldr = ns * nev
@@ -166,7 +228,12 @@ def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: c
         return amul(b)
 
     def prec(b):
-        return matblock(b) if inner_iter == 1 else b
+        # PRECBLOCK (gplmr.F90:466-502): 0 never called, 1 -> MATBLOCK, else BASIC
+        if inner_iter == 0:
+            return b
+        if inner_iter == 1:
+            return matblock(b)
+        return basic_block_lan_solve(matblock, b, sigma, inner_iter)
 
     normalize(x)
     ax = matblock(x)

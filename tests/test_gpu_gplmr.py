@@ -12,7 +12,7 @@ import json
 import numpy as np
 import pytest
 
-from oracle.gplmr import gplmr_eig
+from oracle.gplmr import basic_block_lan_solve, gplmr_eig
 from oracle.hamiltonian import Hamiltonian, exchange, singlet_project
 
 pytestmark = pytest.mark.gpu
@@ -83,24 +83,48 @@ def test_gplmr_matches_oracle_solver(dex, n, nev, sigma, tol, seed):
     assert match(r.e, ref.e) < 1e-9 and match(ref.e, r.e) < 1e-9
 
 
-@pytest.mark.parametrize("nev,inner", [(4, 1), (1, 0), (3, 0)])
+@pytest.mark.parametrize("nev,inner", [(4, 1), (1, 0), (3, 0), (4, 3), (2, 6)])
 def test_iteration_trajectory_matches_oracle(dex, nev, inner):
-    """Step-by-step parity: from the same start block, 15 GPLMR iterations on the
+    """Step-by-step parity: from the same start block, 15 (4 with an inner solve) GPLMR iterations on the
     GPU and in the oracle give the same Rayleigh quotients (the iterations differ
-    only by rounding), including the PRECBLOCK W <- H W form (inner_iter = 1)."""
+    only by rounding), including PRECBLOCK's W <- H W form (inner_iter = 1)
+    and the in-house inner solve (inner_iter >= 2)."""
     print(f"case: trajectory nev={nev} inner={inner}", flush=True)
     h = Hamiltonian(2, 0.5, 0.3)
     v = h.potential2()
     sigma = -0.78 - 0.002j
     x0 = start_block(h, nev, 7)
+    # the inner solve converges within a few iterations: stop while the
+    # residual is still far above rounding level
+    its = 15 if inner < 2 else 4
     ref = gplmr_eig(lambda b: np.stack([h.hammult(b[:, j], v) for j in range(b.shape[1])], axis=1),
-                    x0, sigma, nev, ns=1, maxit=15, tol=1e-14, inner_iter=inner)
+                    x0, sigma, nev, ns=1, maxit=its, tol=1e-14, inner_iter=inner)
     x = x0.copy(order="F")
     with dex.HeliumGrid(2, 0.5, 0.3) as g:
-        r = g.gplmr_eig(x, sigma, nev, ns=1, maxit=15, tol=1e-14, inner_iter=inner)
-    assert r.iterations == ref.iters == 15
+        r = g.gplmr_eig(x, sigma, nev, ns=1, maxit=its, tol=1e-14, inner_iter=inner)
+    assert r.iterations == ref.iters == its
     assert match(r.e, ref.e) < 1e-8 and match(ref.e, r.e) < 1e-8
     assert r.max_residual == pytest.approx(float(ref.resnrm.max()), rel=1e-4)
+
+
+@pytest.mark.parametrize("k,order,sym", [(1, 1, 0), (3, 4, 0), (4, 5, 1), (2, 8, 1)])
+def test_precblock_matches_oracle(dex, k, order, sym):
+    """PRECBLOCK on the GPU (one matvec per Krylov column, block Gram-Schmidt)
+    against the oracle's step-by-step basicblocklansolve on the same block:
+    the same least-squares solution up to rounding."""
+    print(f"case: precblock k={k} order={order} sym={sym}", flush=True)
+    h = Hamiltonian(2, 0.5, 0.3)
+    v = h.potential2()
+    sigma = -0.78 - 0.002j
+    b = start_block(h, k, 31 + order)
+    amul = lambda y: np.stack([h.hammult(y[:, j], v) for j in range(y.shape[1])], axis=1)
+    ref = amul(b) if order == 1 else basic_block_lan_solve(amul, b, sigma, order)
+    with dex.HeliumGrid(2, 0.5, 0.3) as g:
+        xd = torch.from_numpy(np.ascontiguousarray(b.T)).to("cuda")
+        pd = torch.empty_like(xd)
+        g.precblock_device(xd, pd, k, sigma, order, symmetry=sym)
+        got = pd.cpu().numpy().T
+    assert np.linalg.norm(got - ref) <= 1e-9 * np.linalg.norm(ref)
 
 
 def test_mixed_precision_within_1e7(dex):

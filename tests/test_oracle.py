@@ -7,6 +7,8 @@
 * HAMMULT: the einsum restatement against an independently built Kronecker
   Hamiltonian, exchange symmetry and complex symmetry.
 * GPLMR: the fixed point against a dense eigendecomposition.
+* PRECBLOCK_BASIC: the inner solve against a brute-force least-squares solve
+  over the explicit (monomial) block Krylov basis.
 """
 import json
 
@@ -15,7 +17,7 @@ import pytest
 import scipy.sparse as sp
 
 from oracle.hamiltonian import Hamiltonian, exchange, getinverse, kematrix, singlet_project
-from oracle.gplmr import gplmr_eig, orth_block
+from oracle.gplmr import basic_block_lan_solve, gplmr_eig, orth_block
 
 
 def test_kematrix_matches_sinedvr_formula():
@@ -168,3 +170,29 @@ def test_c_oracle_matches_numpy_oracle():
         for j in range(2):
             ref = h.hammult(x[:, j])
             assert np.max(np.abs(y[:, j] - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("order,k", [(1, 1), (1, 3), (2, 2), (4, 2)])
+def test_inner_solve_is_krylov_least_squares(order, k):
+    """basicblocklansolve = argmin over span{b, Hb, ..., H^(m-1) b} of
+    ||(H - sigma) x - b||: checked against numpy lstsq on the explicit
+    monomial basis (an independent route to the same minimiser).  A wrong
+    shift sign, a dropped Gram-Schmidt pass or a skipped column changes x."""
+    h = Hamiltonian(1, 0.6, 0.3)
+    hd = h.dense()
+    sigma = 0.5 - 0.1j
+    rng = np.random.default_rng(40 + order * 10 + k)
+    b = rng.standard_normal((h.d, k)) + 1j * rng.standard_normal((h.d, k))
+    x = basic_block_lan_solve(lambda y: hd @ y, b, sigma, order)
+    krylov = [b]
+    for _ in range(order - 1):
+        krylov.append(hd @ krylov[-1])
+    kmat = np.concatenate(krylov, axis=1)
+    amat = hd @ kmat - sigma * kmat
+    coef = np.linalg.lstsq(amat, b, rcond=None)[0]
+    ref = kmat @ coef
+    assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref)
+    # the residual is orthogonal to (H - sigma) K_m
+    r = hd @ x - sigma * x - b
+    cn = np.linalg.norm(amat, axis=0)
+    assert (np.abs(amat.conj().T @ r) / cn[:, None]).max() <= 1e-9 * np.linalg.norm(b)
