@@ -25,6 +25,9 @@ DEX_EZEROBLOCK = 10
 DEX_FP64 = 0
 DEX_FP32 = 1
 
+# kernel classes of dex_get_kernel_stats
+KERNEL_CLASSES = ("pass1", "pass2", "symadd", "gram", "combine", "coldots", "generic")
+
 # every symbol include/dex.h declares
 EXPORTS = [
     "dex_params_default", "dex_ctx_create", "dex_ctx_destroy", "dex_ctx_info",
@@ -33,6 +36,7 @@ EXPORTS = [
     "dex_dots_device", "dex_solve_opts_default", "dex_gplmr_eig", "dex_gplmr_eig_device",
     "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint", "dex_precblock_device",
     "dex_status_string", "dex_last_error", "dex_set_kernel_timing", "dex_get_kernel_timing",
+    "dex_get_kernel_stats", "dex_reset_kernel_stats",
     "dex_host_kematrix", "dex_host_getinverse", "dex_host_solve_gevp",
 ]
 
@@ -111,6 +115,8 @@ def load() -> C.CDLL:
         "dex_last_error": (C.c_char_p, []),
         "dex_set_kernel_timing": (C.c_int, [P, C.c_int32]),
         "dex_get_kernel_timing": (C.c_int, [P, dp, dp]),
+        "dex_get_kernel_stats": (C.c_int, [P, C.c_int32, dp, C.POINTER(C.c_int64), dp, dp]),
+        "dex_reset_kernel_stats": (C.c_int, [P]),
         "dex_host_kematrix": (C.c_int, [C.c_int32, C.c_double, dp]),
         "dex_host_getinverse": (C.c_int, [C.c_int32, C.c_double, C.c_int32, C.c_int32, dp]),
         "dex_host_solve_gevp": (C.c_int, [C.c_int32, dp, dp, C.c_int32, C.c_int32, dp, dp]),

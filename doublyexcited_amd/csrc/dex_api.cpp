@@ -66,9 +66,74 @@ struct dex_ctx {
   bool timing = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double t_pass1 = -1.0, t_pass2 = -1.0;
+  // per-kernel-class launch timing: event pairs recorded on the stream and
+  // resolved lazily (no synchronisation added to the timed work)
+  struct Pending {
+    hipEvent_t a, b;
+    int cls;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+  double k_ms[DEX_KCLASSES] = {};
+  int64_t k_n[DEX_KCLASSES] = {};
+  double k_bytes[DEX_KCLASSES] = {};  // algorithmic HBM bytes of the timed launches
+  double k_flops[DEX_KCLASSES] = {};  // algorithmic flops of the timed launches
 };
 
 namespace {
+
+hipEvent_t take_event(dex_ctx *c) {
+  if (!c->free_events.empty()) {
+    hipEvent_t e = c->free_events.back();
+    c->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Accumulates every recorded pair (waits for the last one).
+hipError_t resolve_timing(dex_ctx *c) {
+  if (c->pending.empty()) return hipSuccess;
+  hipError_t e = hipEventSynchronize(c->pending.back().b);
+  for (const auto &pd : c->pending) {
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, pd.a, pd.b);
+    c->k_ms[pd.cls] += ms;
+    c->k_n[pd.cls] += 1;
+    c->free_events.push_back(pd.a);
+    c->free_events.push_back(pd.b);
+  }
+  c->pending.clear();
+  return e;
+}
+
+// Brackets the launches of one kernel class with events when timing is on.
+// `bytes` and `flops` are the launch's algorithmic HBM traffic and arithmetic.
+struct KTimer {
+  dex_ctx *c;
+  int cls;
+  hipEvent_t a = nullptr;
+  KTimer(dex_ctx *c_, int cls_, double bytes, double flops) : c(c_), cls(cls_) {
+    if (c->timing && (a = take_event(c)) != nullptr) {
+      (void)hipEventRecord(a, c->stream);
+      c->k_bytes[cls] += bytes;
+      c->k_flops[cls] += flops;
+    }
+  }
+  ~KTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) {
+      c->free_events.push_back(a);
+      return;
+    }
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({a, b, cls});
+    if (c->pending.size() >= 8192) (void)resolve_timing(c);
+  }
+};
 
 int ensure_u(dex_ctx *c) {
   if (c->d_u) return 0;
@@ -94,12 +159,25 @@ int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0) {
   if (fast) {
     const int io = single ? dex::kIoMixed : dex::kIoF64;
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
-    DEX_HIP(dex::launch_pass1(c->nn, io, x, c->d_u, s));
+    const double dd = double(c->d), nn = double(c->nn);
+    const double ub = single ? 8.0 : 16.0;  // intermediate U entry bytes
+    {
+      // reads X, writes U; three axes of NN real-by-complex MACs per entry
+      KTimer kt(c, DEX_K_PASS1, (16.0 + ub) * dd, 12.0 * nn * dd);
+      DEX_HIP(dex::launch_pass1(c->nn, io, x, c->d_u, s));
+    }
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
-    DEX_HIP(second_pass(c, io, x, y, sym));
+    {
+      // reads U and X, writes Y; the general pass also applies the three
+      // electron-2 axes, the transpose-add pass the exchange image
+      const double fl = sym != 0 ? 24.0 * dd : (12.0 * nn + 14.0) * dd;
+      KTimer kt(c, sym != 0 ? DEX_K_SYMADD : DEX_K_PASS2, (32.0 + ub) * dd, fl);
+      DEX_HIP(second_pass(c, io, x, y, sym));
+    }
     if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
     return 0;
   }
+  KTimer kt(c, DEX_K_GENERIC, 32.0 * double(c->d), (24.0 * c->nn + 14.0) * double(c->d));
   DEX_HIP(dex::launch_matvec_generic(c->nn, x, y, c->d_vn, c->d_tinv, c->coef, s));
   return 0;
 }
@@ -122,14 +200,30 @@ int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
   return 0;
 }
 
+// Distinct vectors the descriptors read (a, and b where shifted).
+double distinct_vectors(const std::vector<ColDesc> &ds, size_t n) {
+  std::vector<const void *> v;
+  for (size_t i = 0; i < n && i < ds.size(); ++i) {
+    if (ds[i].a) v.push_back(ds[i].a);
+    if (ds[i].b) v.push_back(ds[i].b);
+  }
+  std::sort(v.begin(), v.end());
+  return double(std::unique(v.begin(), v.end()) - v.begin());
+}
+
 // Runs one Gram launch (kl <= 64, kr <= 128) and copies the kl x kr result
 // into g (leading dimension ldg) at (i0, j0).  With share > 0 the skipped lower
 // blocks of the Hermitian share x share part are filled by conjugate symmetry.
 int gram_launch_copy(dex_ctx *c, const std::vector<ColDesc> &ds, int kl, int kr, int share,
                      std::vector<cd> &g, int ldg, int i0, int j0) {
   if (int st = upload_descs(c, ds)) return st;
-  DEX_HIP(dex::launch_gram(c->d_desc, kl, kr, share, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
-                           c->d_gram_out, c->stream));
+  {
+    const double hskip = share > 0 ? 0.5 * double(share) * double(share - 1) : 0.0;
+    KTimer kt(c, DEX_K_GRAM, 16.0 * double(c->d) * distinct_vectors(ds, ds.size()),
+              8.0 * double(c->d) * (double(kl) * kr - hskip));
+    DEX_HIP(dex::launch_gram(c->d_desc, kl, kr, share, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
+                             c->d_gram_out, c->stream));
+  }
   std::vector<cd> part(size_t(kl) * kr);
   DEX_HIP(hipMemcpyAsync(part.data(), c->d_gram_out, part.size() * sizeof(Dbl2), hipMemcpyDeviceToHost,
                          c->stream));
@@ -137,7 +231,7 @@ int gram_launch_copy(dex_ctx *c, const std::vector<ColDesc> &ds, int kl, int kr,
   for (int j = 0; j < kr; ++j)
     for (int i = 0; i < kl; ++i) {
       cd v = part[i + size_t(kl) * j];
-      if (share > 0 && i < share && j < share && (i / 16) > (j / 16)) v = std::conj(part[j + size_t(kl) * i]);
+      if (share > 0 && (i / 16) > (j / 16) && 16 * (i / 16 + 1) <= share) v = std::conj(part[j + size_t(kl) * i]);
       g[(i0 + i) + size_t(ldg) * (j0 + j)] = v;
     }
   return 0;
@@ -198,6 +292,9 @@ int combine(dex_ctx *c, const std::vector<ColDesc> &in, const std::vector<Dbl2 *
     DEX_HIP(hipMemcpyAsync(c->d_small2, beta->data(), beta->size() * sizeof(Dbl2),
                            hipMemcpyHostToDevice, c->stream));
   }
+  KTimer kt(c, DEX_K_COMBINE,
+            16.0 * double(c->d) * (distinct_vectors(in, in.size()) + double(nout) * (beta ? 2.0 : 1.0)),
+            8.0 * double(c->d) * nin * nout);
   DEX_HIP(dex::launch_combine(c->d_desc, nin, nout, c->d_small, beta ? c->d_small2 : nullptr, c->d,
                               c->stream));
   return 0;
@@ -214,8 +311,12 @@ int coldots(dex_ctx *c, const std::vector<ColDesc> &a, const std::vector<ColDesc
     ds.insert(ds.end(), b.begin(), b.end());
   }
   if (int st = upload_descs(c, ds)) return st;
-  DEX_HIP(dex::launch_coldots(c->d_desc, k, c->d, reinterpret_cast<double *>(c->d_partial), c->d_dots,
-                              c->stream));
+  {
+    KTimer kt(c, DEX_K_COLDOTS, 16.0 * double(c->d) * distinct_vectors(ds, ds.size()),
+              (b.empty() ? 4.0 : 16.0) * double(c->d) * k);
+    DEX_HIP(dex::launch_coldots(c->d_desc, k, c->d, reinterpret_cast<double *>(c->d_partial), c->d_dots,
+                                c->stream));
+  }
   std::vector<double> h(size_t(k) * 4);
   DEX_HIP(hipMemcpyAsync(h.data(), c->d_dots, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   DEX_HIP(hipStreamSynchronize(c->stream));
@@ -362,6 +463,11 @@ int dex_ctx_destroy(dex_ctx *c) {
     if (ptr) (void)hipFree(ptr);
   for (auto &ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (const auto &pd : c->pending) {
+    (void)hipEventDestroy(pd.a);
+    (void)hipEventDestroy(pd.b);
+  }
+  for (hipEvent_t ev : c->free_events) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DEX_OK;
@@ -395,6 +501,30 @@ int dex_get_tinv(const dex_ctx *c, double *tinv) {
 int dex_set_kernel_timing(dex_ctx *c, int32_t enable) {
   if (!c) return fail(DEX_EINVAL, "null argument");
   c->timing = enable != 0;
+  return DEX_OK;
+}
+
+int dex_get_kernel_stats(dex_ctx *c, int32_t kclass, double *total_ms, int64_t *launches,
+                         double *alg_bytes, double *alg_flops) {
+  if (!c || !total_ms || !launches || !alg_bytes || !alg_flops) return fail(DEX_EINVAL, "null argument");
+  if (kclass < 0 || kclass >= DEX_KCLASSES) return fail(DEX_EINVAL, "unknown kernel class");
+  DEX_HIP(hipSetDevice(c->p.device));
+  DEX_HIP(resolve_timing(c));
+  *total_ms = c->k_ms[kclass];
+  *launches = c->k_n[kclass];
+  *alg_bytes = c->k_bytes[kclass];
+  *alg_flops = c->k_flops[kclass];
+  return DEX_OK;
+}
+
+int dex_reset_kernel_stats(dex_ctx *c) {
+  if (!c) return fail(DEX_EINVAL, "null argument");
+  DEX_HIP(hipSetDevice(c->p.device));
+  DEX_HIP(resolve_timing(c));
+  for (int i = 0; i < DEX_KCLASSES; ++i) {
+    c->k_ms[i] = c->k_bytes[i] = c->k_flops[i] = 0.0;
+    c->k_n[i] = 0;
+  }
   return DEX_OK;
 }
 
@@ -685,79 +815,302 @@ struct Solver {
     return 0;
   }
 
+  // G = [w; q]^H w ((k + kq) x k, rows w then q): one pass over w and q
+  // (q in chunks of up to 64 columns per launch beyond the first).
+  int gram_wq(const std::vector<ColDesc> &w, const Block &q, std::vector<cd> &g) {
+    const int k = int(w.size()), kq = q.size(), rows = k + kq;
+    if (k < 1 || k > 48) return fail(DEX_EINVAL, "block width must be 1..48");
+    g.assign(size_t(rows) * k, 0.0);
+    const double t0 = now_s();
+    const int n1 = std::min(kq, 64 - k);
+    std::vector<ColDesc> ds(w);
+    for (int i = 0; i < n1; ++i) ds.push_back(plain(q.cols[i]));
+    if (int st = gram_launch_copy(c, ds, k + n1, k, k, g, rows, 0, 0)) return st;
+    for (int i0 = n1; i0 < kq; i0 += 64) {
+      const int bl = std::min(64, kq - i0);
+      std::vector<ColDesc> d2;
+      for (int i = 0; i < bl; ++i) d2.push_back(plain(q.cols[i0 + i]));
+      d2.insert(d2.end(), w.begin(), w.end());
+      if (int st = gram_launch_copy(c, d2, bl, k, 0, g, rows, k + i0, 0)) return st;
+    }
+    t_gram += now_s() - t0;
+    return 0;
+  }
+
+  // out = [w q] [T; -Cq T] (k x k T, kq x k Cq), written over the columns of
+  // `out` (which may be w's own storage: the combine is row-local and w is read
+  // in the first launch only).
+  int apply_wq(const std::vector<ColDesc> &w, const Block &q, const std::vector<cd> &t,
+               const std::vector<cd> &cq, Block &out) {
+    const int k = int(w.size()), kq = q.size();
+    const double t0 = now_s();
+    std::vector<cd> one(k, 1.0);
+    // -Cq T
+    std::vector<cd> ct(size_t(kq) * k, 0.0);
+    for (int j = 0; j < k; ++j)
+      for (int l = 0; l < k; ++l) {
+        const cd tlj = t[l + size_t(k) * j];
+        if (tlj == cd(0.0))
+          continue;
+        for (int i = 0; i < kq; ++i) ct[i + size_t(kq) * j] -= cq[i + size_t(kq) * l] * tlj;
+      }
+    for (int q0 = 0; q0 == 0 || q0 < kq;) {
+      const bool first = q0 == 0;
+      const int nq = std::min(first ? 64 - k : 64, kq - q0);
+      const int nin = (first ? k : 0) + nq;
+      const int off = first ? k : 0;
+      std::vector<cd> cm(size_t(nin) * k, 0.0);
+      for (int j = 0; j < k; ++j) {
+        if (first)
+          for (int i = 0; i < k; ++i) cm[i + size_t(nin) * j] = t[i + size_t(k) * j];
+        for (int i = 0; i < nq; ++i) cm[off + i + size_t(nin) * j] = ct[q0 + i + size_t(kq) * j];
+      }
+      std::vector<ColDesc> in;
+      if (first) in = w;
+      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i]));
+      if (int st = combine(c, in, out.cols, cm, first ? nullptr : &one)) return st;
+      q0 += nq;
+      if (nq == 0) break;
+    }
+    t_update += now_s() - t0;
+    return 0;
+  }
+
+  // Projected Gram P = W^H W - Cq^H Cq of (I - q q^H) w from g = [w; q]^H w,
+  // and the smallest ratio P_jj / (W^H W)_jj (the fraction of a column's norm
+  // squared that survives the projection).
+  static void projected_gram(const std::vector<cd> &g, int k, int kq, std::vector<cd> &p,
+                             double &min_keep) {
+    const int rows = k + kq;
+    p.assign(size_t(k) * k, 0.0);
+    min_keep = 1.0;
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < k; ++i) {
+        cd v = g[i + size_t(rows) * j];
+        for (int l = 0; l < kq; ++l) v -= std::conj(g[k + l + size_t(rows) * i]) * g[k + l + size_t(rows) * j];
+        p[i + size_t(k) * j] = v;
+      }
+    for (int j = 0; j < k; ++j) {
+      const double w2 = g[j + size_t(rows) * j].real();
+      const double keep = w2 > 0.0 ? p[j + size_t(k) * j].real() / w2 : 0.0;
+      min_keep = std::min(min_keep, keep);
+    }
+  }
+
+  // Upper Cholesky factor R of the Hermitian p and its inverse, when every
+  // pivot keeps more than 1e-12 of the corresponding column's norm squared
+  // (w2: those norms); false otherwise.
+  static bool chol_inverse(const std::vector<cd> &p, const std::vector<double> &w2, int k, std::vector<cd> &r,
+                           std::vector<cd> &rinv) {
+    r = p;
+    if (dex::zpotrf_upper(k, r.data(), k) != 0) return false;
+    for (int i = 0; i < k; ++i) {
+      const double piv = r[i + size_t(k) * i].real();
+      if (!(piv > 0.0) || !(piv * piv > 1e-12 * w2[i])) return false;
+    }
+    rinv = r;
+    dex::ztrtri_upper(k, rinv.data(), k);
+    for (int j = 0; j < k; ++j)
+      for (int i = j + 1; i < k; ++i) r[i + size_t(k) * j] = 0.0;
+    return true;
+  }
+
+  // Block classical Gram-Schmidt with cholQR in one pass: w <- (w - q C) R^{-1}
+  // with C = q^H w and R^H R = w^H w - C^H C, i.e. one Gram [w; q]^H w and one
+  // combine over [w q].  A second pass runs when the projection removed more
+  // than half of some column's norm squared (the criterion of Daniel, Gragg,
+  // Kaufman and Stewart, Math. Comp. 30 (1976) 772-795).  q must be
+  // orthonormal.  The input is `wd` (descriptors: plain columns or a - sigma b),
+  // the result goes to `w` (which may be the storage wd reads).  On success
+  // `cq` and `r` satisfy w_in = q cq + w_out r; `ok` is false when the
+  // projected block is numerically rank deficient (then w holds the projected,
+  // unnormalised block and the caller decides).  `materialize` false skips the
+  // final combine (the Arnoldi step that only needs C and R).
+  int cgs_cholqr(const Block &q, const std::vector<ColDesc> &wd, Block &w, std::vector<cd> &cq,
+                 std::vector<cd> &r, bool &ok, bool materialize = true) {
+    const int k = int(wd.size()), kq = q.size();
+    std::vector<cd> g, p, rinv;
+    double keep = 1.0;
+    if (int st = gram_wq(wd, q, g)) return st;
+    double t0 = now_s();
+    projected_gram(g, k, kq, p, keep);
+    std::vector<double> w2(k);
+    for (int j = 0; j < k; ++j) w2[j] = g[j + size_t(k + kq) * j].real();
+    cq.assign(size_t(kq) * k, 0.0);
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < kq; ++i) cq[i + size_t(kq) * j] = g[k + i + size_t(k + kq) * j];
+    bool good = keep > 0.5 && chol_inverse(p, w2, k, r, rinv);
+    t_small += now_s() - t0;
+    if (good) {
+      ok = true;
+      if (!materialize) return 0;
+      return apply_wq(wd, q, rinv, cq, w);
+    }
+    // second pass on the projected block
+    std::vector<cd> id(size_t(k) * k, 0.0);
+    for (int j = 0; j < k; ++j) id[j + size_t(k) * j] = 1.0;
+    if (int st = apply_wq(wd, q, id, cq, w)) return st;
+    if (kq == 0) {
+      ok = false;
+      return 0;
+    }
+    std::vector<cd> g2, cq2;
+    if (int st = gram_wq(descs_of(w), q, g2)) return st;
+    t0 = now_s();
+    projected_gram(g2, k, kq, p, keep);
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < kq; ++i) cq[i + size_t(kq) * j] += g2[k + i + size_t(k + kq) * j];
+    cq2.assign(size_t(kq) * k, 0.0);
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < kq; ++i) cq2[i + size_t(kq) * j] = g2[k + i + size_t(k + kq) * j];
+    good = keep > 0.5 && chol_inverse(p, w2, k, r, rinv);
+    t_small += now_s() - t0;
+    if (!good) {
+      ok = false;
+      if (int st = apply_wq(descs_of(w), q, id, cq2, w)) return st;
+      return 0;
+    }
+    ok = true;
+    if (!materialize) return 0;
+    return apply_wq(descs_of(w), q, rinv, cq2, w);
+  }
+
+  // w <- orthonormal basis of (I - q q^H) w (q orthonormal), in place, with
+  // the optional image block aw transformed alike (aq being q's image).  The
+  // fused pass above; where it reports rank deficiency, the projection is
+  // completed and the rank-revealing cholQR (orth) takes over, dropping
+  // dependent directions when `drop`.
+  int orth_against(const Block &q, Block &w, const Block *aq, Block *aw, bool drop) {
+    if (w.size() == 0) return 0;
+    if (aw == nullptr) {
+      std::vector<cd> cq, r;
+      bool ok = false;
+      if (int st = cgs_cholqr(q, descs_of(w), w, cq, r, ok)) return st;
+      if (ok) return 0;
+      const int kk = orth(w, nullptr, drop);
+      return kk < 0 ? -kk : 0;
+    }
+    // with an image block: the same coefficients for both
+    const int k = w.size(), kq = q.size();
+    std::vector<cd> g, p, r, rinv;
+    double keep = 1.0;
+    if (int st = gram_wq(descs_of(w), q, g)) return st;
+    const double t0 = now_s();
+    projected_gram(g, k, kq, p, keep);
+    std::vector<double> w2(k);
+    for (int j = 0; j < k; ++j) w2[j] = g[j + size_t(k + kq) * j].real();
+    std::vector<cd> cq(size_t(kq) * k);
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < kq; ++i) cq[i + size_t(kq) * j] = g[k + i + size_t(k + kq) * j];
+    const bool good = keep > 0.5 && chol_inverse(p, w2, k, r, rinv);
+    t_small += now_s() - t0;
+    if (good) {
+      if (int st = apply_wq(descs_of(*aw), *aq, rinv, cq, *aw)) return st;
+      return apply_wq(descs_of(w), q, rinv, cq, w);
+    }
+    if (kq > 0) {
+      if (int st = project_out(q, aq, w, aw)) return st;
+      if (int st = project_out(q, aq, w, aw)) return st;
+    }
+    const int kk = orth(w, aw, drop);
+    return kk < 0 ? -kk : 0;
+  }
+
   // PRECBLOCK_BASIC -> basicblocklansolve (gplmr.F90:674-770;
   // harmonic_ritz_small.F90:258-324): b <- argmin over x in K_m(H, b) of
   // ||(H - sigma) x - b||, the fixed-order unrestarted block minimal-residual
   // approximation to (H - sigma)^{-1} b, for every column of the block (the
   // reference's allhmultsub multiplies only the first, gplmr.F90:714).
   //
-  // Same Krylov space and the same least-squares solution as the reference,
-  // with one matvec per basis column instead of two: the reference forms
-  // V_j = H V_{j-1} by a matvec and then B_j = (H - sigma) V_j by another
-  // (harmonic_ritz_small.F90:122-127); here V_j = B_{j-1} + sigma V_{j-1},
-  // since every transform applied to B is applied to V as well.  B is made
-  // orthonormal block by block (two projections against the earlier blocks,
-  // as myhgramschmidt_many's two passes, then cholQR), so the solution is
-  // x = V (B^H b).
+  // Same Krylov space (K_m(H, b) = K_m(H - sigma, b)) and the same
+  // least-squares minimiser as the reference, computed as block GMRES: an
+  // orthonormal Arnoldi basis Q of K_m(H - sigma, b) with
+  // (H - sigma) Q_m = Q_{m+1} Hbar, one matvec per basis column, one fused
+  // projection/cholQR pass per block, x = Q_m y with
+  // y = argmin ||[R0; 0] - Hbar y|| (b = Q_1 R0).  The reference instead keeps
+  // both V and B = (H - sigma) V, two matvecs per column and two Gram-Schmidt
+  // passes over both (harmonic_ritz_small.F90:100-155).
   int inner_solve(Block &b, int m, cd sigma) {
     const int k = b.size();
     if (k == 0 || m < 1) return 0;
     const double t0 = now_s();
-    Block vall, ball;
-    Block vj, bj;
-    if (int st = grow(vj, k)) return st;
-    if (int st = grow(bj, k)) return st;
-    {
-      std::vector<cd> id(size_t(k) * k, 0.0);
-      for (int j = 0; j < k; ++j) id[j + size_t(k) * j] = 1.0;
-      if (int st = combine(c, descs_of(b), vj.cols, id)) return st;  // V_1 = b
-    }
-    for (int it = 0; it < m; ++it) {
-      if (it > 0) {
-        // V_j = H V_{j-1} = B_{j-1} + sigma V_{j-1}
-        const int kp = vj.size();
-        Block vn, bn;
-        if (int st = grow(vn, kp)) return st;
-        if (int st = grow(bn, kp)) return st;
-        std::vector<cd> cm(size_t(2 * kp) * kp, 0.0);
-        for (int j = 0; j < kp; ++j) {
-          cm[j + size_t(2 * kp) * j] = 1.0;
-          cm[kp + j + size_t(2 * kp) * j] = sigma;
-        }
-        std::vector<ColDesc> in = descs_of(bj);
-        for (Dbl2 *p : vj.cols) in.push_back(plain(p));
-        if (int st = combine(c, in, vn.cols, cm)) return st;
-        vj = vn;
-        bj = bn;
-      }
-      // B_j = (H - sigma) V_j
-      if (int st = matblock(vj, bj)) return st;
-      {
-        const int kj = vj.size();
-        std::vector<cd> cm(size_t(kj) * kj, 0.0);
-        for (int j = 0; j < kj; ++j) cm[j + size_t(kj) * j] = 1.0;
-        std::vector<ColDesc> in;
-        for (int j = 0; j < kj; ++j) in.push_back(shifted(bj.cols[j], vj.cols[j], sigma));
-        if (int st = combine(c, in, bj.cols, cm)) return st;
-      }
-      for (int pass = 0; pass < 2 && ball.size() > 0; ++pass)
-        if (int st = project_out(ball, &vall, bj, &vj)) return st;
-      const int kk = orth(bj, &vj, true);
+    Block qall;
+    // Q_1 R0 = b (into fresh storage: b is overwritten only by the solution)
+    Block q1;
+    if (int st = grow(q1, k)) return st;
+    std::vector<cd> cq, r0;
+    bool ok = false;
+    if (int st = cgs_cholqr(Block{}, descs_of(b), q1, cq, r0, ok)) return st;
+    int kb = k;  // Arnoldi block width
+    if (!ok) {
+      // rank-deficient right-hand side (q1 holds a copy of b): an orthonormal
+      // basis of its range, and R0 = Q1^H b
+      const int kk = orth(q1, nullptr, true);
       if (kk < 0) return -kk;
-      if (bj.size() == 0) break;  // the Krylov space is exhausted
-      ball.cols.insert(ball.cols.end(), bj.cols.begin(), bj.cols.end());
-      vall.cols.insert(vall.cols.end(), vj.cols.begin(), vj.cols.end());
+      kb = q1.size();
+      if (kb == 0) {
+        t_prec += now_s() - t0;
+        return 0;  // b = 0: x = 0, which b already holds
+      }
+      const double tg = now_s();
+      if (int st = gram(c, descs_of(q1), descs_of(b), r0)) return st;
+      t_gram += now_s() - tg;
     }
-    // x = V (B^H b), written over b
-    std::vector<cd> y;
-    double tg = now_s();
-    if (int st = gram(c, descs_of(ball), descs_of(b), y)) return st;
-    t_gram += now_s() - tg;
-    tg = now_s();
-    if (int st = combine_chunked(vall, y, b)) return st;
-    t_update += now_s() - tg;
-    for (Dbl2 *p : vall.cols) pool.push_back(p);
-    for (Dbl2 *p : ball.cols) pool.push_back(p);
+    qall = q1;
+    // Hbar by block columns: column block j holds C (rows of all earlier
+    // blocks) and R_j (the new block's rows)
+    std::vector<std::vector<cd>> hc;
+    std::vector<std::vector<cd>> hr;
+    int nblk = 0;
+    for (int j = 0; j < m; ++j) {
+      Block qj;
+      qj.cols.assign(qall.cols.end() - kb, qall.cols.end());
+      Block hq;
+      if (int st = grow(hq, kb)) return st;
+      if (int st = matblock(qj, hq)) return st;
+      std::vector<ColDesc> wd;
+      for (int i = 0; i < kb; ++i) wd.push_back(shifted(hq.cols[i], qj.cols[i], sigma));
+      std::vector<cd> cj, rj;
+      const bool last = j == m - 1;
+      if (int st = cgs_cholqr(qall, wd, hq, cj, rj, ok, !last)) return st;
+      if (!ok) rj.assign(size_t(kb) * kb, 0.0);  // (H - sigma) Q_j lies in span(Q): the space is exhausted
+      hc.push_back(cj);
+      hr.push_back(rj);
+      ++nblk;
+      if (!ok || last) {
+        for (Dbl2 *p : hq.cols) pool.push_back(p);
+        break;
+      }
+      qall.cols.insert(qall.cols.end(), hq.cols.begin(), hq.cols.end());
+    }
+    // least squares on the (nblk + 1) k x nblk k block Hessenberg matrix
+    const double ts = now_s();
+    const int rows = (nblk + 1) * kb, cols = nblk * kb;
+    std::vector<cd> hb(size_t(rows) * cols, 0.0), rhs(size_t(rows) * k, 0.0);
+    for (int j = 0; j < nblk; ++j) {
+      const int kq = (j + 1) * kb;  // rows of C for block column j
+      for (int cc = 0; cc < kb; ++cc) {
+        for (int i = 0; i < kq; ++i) hb[i + size_t(rows) * (j * kb + cc)] = hc[j][i + size_t(kq) * cc];
+        for (int i = 0; i < kb; ++i) hb[kq + i + size_t(rows) * (j * kb + cc)] = hr[j][i + size_t(kb) * cc];
+      }
+    }
+    for (int cc = 0; cc < k; ++cc)
+      for (int i = 0; i < kb; ++i) rhs[i + size_t(rows) * cc] = r0[i + size_t(kb) * cc];
+    const int lst = dex::zlstsq(rows, cols, hb.data(), rows, rhs.data(), rows, k);
+    t_small += now_s() - ts;
+    if (lst != 0) {
+      for (Dbl2 *p : qall.cols) pool.push_back(p);
+      return fail(DEX_EGEVP, "inner solve: projected least-squares problem is singular");
+    }
+    std::vector<cd> y(size_t(cols) * k);
+    for (int cc = 0; cc < k; ++cc)
+      for (int i = 0; i < cols; ++i) y[i + size_t(cols) * cc] = rhs[i + size_t(rows) * cc];
+    Block qm;
+    qm.cols.assign(qall.cols.begin(), qall.cols.begin() + cols);
+    const double tu = now_s();
+    if (int st = combine_chunked(qm, y, b)) return st;
+    t_update += now_s() - tu;
+    for (Dbl2 *p : qall.cols) pool.push_back(p);
     t_prec += now_s() - t0;
     return 0;
   }
@@ -916,14 +1269,16 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     std::vector<cd> eact;
     for (int j : active) eact.push_back(e[j]);
     // CHOLQR(X); AX <- AX R^{-1}  (:88-90)
-    if (int st = S.orth(X, &AX, false)) return -st;
+    {
+      Block none, anone;
+      if (int st = S.orth_against(none, X, &anone, &AX, false)) return st;
+    }
     if (X.size() != nev) return fail(DEX_ECHOLESKY, "X lost rank");
     // PRECBLOCK (:92-99): GM_MAX_INNER_ITER = 1 gives W <- H W (:491-493),
     // larger values the inner minimal-residual solve
     if (int st = precblock(W)) return st;
     // W <- W - X (X^H W); CHOLQR(W)  (:101-105)
-    if (int st = S.project_out(X, nullptr, W, nullptr)) return st;
-    if (int st = S.orth(W, nullptr, true)) return -st;
+    if (int st = S.orth_against(X, W, nullptr, nullptr, true)) return st;
     AW.cols.clear();
     if (int st = S.grow(AW, W.size())) return st;
     if (int st = S.matblock(W, AW)) return st;  // :108
@@ -949,10 +1304,13 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
         if (int st = combine(c, in, sl.cols, cm)) return st;
       }
       if (int st = precblock(sl)) return st;  // :118-126, :148-156
-      if (int st = S.project_out(X, nullptr, sl, nullptr)) return st;
-      if (int st = S.project_out(W, nullptr, sl, nullptr)) return st;
-      if (int st = S.project_out(Sblk, nullptr, sl, nullptr)) return st;
-      if (int st = S.orth(sl, nullptr, true)) return -st;
+      {
+        // X, W and the earlier S blocks are mutually orthonormal: one projection
+        Block xws = X;
+        xws.cols.insert(xws.cols.end(), W.cols.begin(), W.cols.end());
+        xws.cols.insert(xws.cols.end(), Sblk.cols.begin(), Sblk.cols.end());
+        if (int st = S.orth_against(xws, sl, nullptr, nullptr, true)) return st;
+      }
       Block asl;
       if (int st = S.grow(asl, sl.size())) return st;
       if (int st = S.matblock(sl, asl)) return st;
@@ -963,10 +1321,12 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     }
     // P against X, W, S; CHOLQR(P); AP <- AP R^{-1}  (:177-196)
     if (iter > 1 && P.size() > 0) {
-      if (int st = S.project_out(X, &AX, P, &AP)) return st;
-      if (int st = S.project_out(W, &AW, P, &AP)) return st;
-      if (int st = S.project_out(Sblk, &ASblk, P, &AP)) return st;
-      if (int st = S.orth(P, &AP, true)) return -st;
+      Block q = X, aq = AX;
+      q.cols.insert(q.cols.end(), W.cols.begin(), W.cols.end());
+      q.cols.insert(q.cols.end(), Sblk.cols.begin(), Sblk.cols.end());
+      aq.cols.insert(aq.cols.end(), AW.cols.begin(), AW.cols.end());
+      aq.cols.insert(aq.cols.end(), ASblk.cols.begin(), ASblk.cols.end());
+      if (int st = S.orth_against(q, P, &aq, &AP, true)) return st;
     }
     // Harmonic projection K = (AQ - sigma Q)^H (AQ - sigma Q), M = (AQ - sigma Q)^H Q (:201-277)
     Block Q, AQ;

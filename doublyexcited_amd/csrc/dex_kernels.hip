@@ -556,7 +556,18 @@ hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *v
 namespace {
 
 constexpr int kGramThreads = 256;
-constexpr int kGramRows = 16;        // rows per LDS tile
+#ifndef DEX_GRAM_ROWS
+#define DEX_GRAM_ROWS 16
+#endif
+constexpr int kGramRowsMax = DEX_GRAM_ROWS;  // rows per LDS tile (at most)
+
+// Rows per LDS tile for an (MA, NB) Gram: the largest power of two <=
+// kGramRowsMax whose tile fits 48 KB (three workgroups per CU), at least 16.
+constexpr int gram_rows(int ma, int nb) {
+  int r = kGramRowsMax;
+  while (r > 16 && r * (16 * ma + 16 * nb + 1) * 16 > 48 * 1024) r /= 2;
+  return r;
+}
 constexpr int kGramMaxL = 64;
 constexpr int kGramMaxR = 128;
 constexpr int kGramBlocks = 1024;
@@ -577,14 +588,15 @@ __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
 // FMAs and consecutive lanes read consecutive columns.  LDS holds kGramRows
 // rows: L in slots [0, 16 MA), the R columns not shared with L after it.
 // R column j < share is L column j.  For share > 0 the blocks (m, n) with
-// m > n and 16 n < share lie below the diagonal of the Hermitian L^H L part
-// and are skipped.
+// m > n that lie wholly inside the leading share x share part (Hermitian:
+// L^H L there) are skipped.
 template <int MA, int NB>
 __global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict__ descs, int kl,
                                                        int kr, int share, int64_t d,
                                                        int64_t rows_per_block,
                                                        double2 *__restrict__ partial) {
   constexpr int LD = 16 * MA + 16 * NB + 1;
+  constexpr int kGramRows = gram_rows(MA, NB);
   __shared__ double2 tile[kGramRows * LD];
   const int tid = threadIdx.x;
   const int a = tid % 16, b = tid / 16;
@@ -639,7 +651,7 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict
       for (int m = 0; m < MA; ++m)
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
-          if (m > n && 16 * n < share) continue;
+          if (m > n && 16 * (m + 1) <= share) continue;
           accr[m][n] += lv[m].x * rv[n].x + lv[m].y * rv[n].y;
           acci[m][n] += lv[m].x * rv[n].y - lv[m].y * rv[n].x;
         }
@@ -654,6 +666,140 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict
       const int i = a + 16 * m, j = b + 16 * n;
       if (i < kl && j < kr) out[i + size_t(kl) * j] = make_double2(accr[m][n], acci[m][n]);
     }
+}
+
+// The same partial Gram on the FP64 matrix cores.  G = L^H R is a GEMM whose
+// inner dimension is the vector length: per 4 rows, v_mfma_f64_16x16x4f64
+// multiplies a 16 x 4 slice of L^T by a 4 x 16 slice of R into a 16 x 16
+// output tile (lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and
+// holds D[4 v + l / 16][l % 16], v < 4).  A complex tile takes four real
+// products: Re += Lr Rr + Li Ri, Im += Lr Ri - Li Rr.  The rows are staged
+// through LDS as in k_gram (one coalesced pass over every column); each LDS
+// value then feeds 16 MACs of a matrix instruction instead of one vector FMA,
+// which is what limited k_gram to the LDS bandwidth.  The 4 waves of a
+// workgroup share the output tiles (MT x NT of them) round-robin.
+typedef double dex_v4d __attribute__((ext_vector_type(4)));
+
+constexpr int gram_mfma_rows(int mt, int nt) {
+  int r = 32;
+  while (r > 4 && r * (16 * mt + 16 * nt + 1) * 16 > 48 * 1024) r /= 2;
+  return r;
+}
+
+template <int MT, int NT>
+__global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__restrict__ descs, int kl,
+                                                            int kr, int share, int64_t d,
+                                                            int64_t rows_per_block,
+                                                            double2 *__restrict__ partial) {
+  constexpr int LD = 16 * MT + 16 * NT + 1;
+  constexpr int ROWS = gram_mfma_rows(MT, NT);
+  constexpr int QMAX = (MT * NT + 3) / 4;
+  // loads per thread per tile: every thread issues all of them before any
+  // wait, and the next tile's while the matrix cores work on this one
+  constexpr int PMAX = (ROWS * (16 * MT + 16 * NT) + kGramThreads - 1) / kGramThreads;
+  __shared__ double2 tile[ROWS * LD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nload = kl + (kr - share);
+  const int mt_valid = (kl + 15) / 16, nt_valid = (kr + 15) / 16;
+  int tm[QMAX], tn[QMAX];
+  bool tv[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    const int t = wave + 4 * q;
+    tm[q] = t % MT;
+    tn[q] = t / MT;
+    tv[q] = t < MT * NT && tm[q] < mt_valid && tn[q] < nt_valid &&
+            !(tm[q] > tn[q] && 16 * (tm[q] + 1) <= share);
+  }
+  dex_v4d accr[QMAX], acci[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    accr[q] = dex_v4d{0.0, 0.0, 0.0, 0.0};
+    acci[q] = dex_v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  // Element p of this thread is (row e % ROWS, column e / ROWS) of every
+  // tile, e = tid + 256 p: its column pointers are fixed for the whole loop.
+  const double2 *pa[PMAX];
+  const double2 *pb[PMAX];
+#pragma unroll
+  for (int p = 0; p < PMAX; ++p) {
+    const int e = tid + kGramThreads * p;
+    pa[p] = nullptr;
+    pb[p] = nullptr;
+    if (e < ROWS * nload) {
+      const ColDesc &dc = descs[e / ROWS];
+      pa[p] = reinterpret_cast<const double2 *>(dc.a);
+      pb[p] = reinterpret_cast<const double2 *>(dc.b);
+    }
+  }
+  const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
+  double2 va[PMAX], vb[PMAX];
+  auto fetch = [&](int64_t rb) {
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+      const int64_t row = rb + (tid + kGramThreads * p) % ROWS;
+      va[p] = make_double2(0.0, 0.0);
+      vb[p] = make_double2(0.0, 0.0);
+      if (pa[p] != nullptr && row < r1) {
+        va[p] = pa[p][row];
+        if (pb[p] != nullptr) vb[p] = pb[p][row];
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+      if (pa[p] == nullptr) continue;  // beyond the loads, or supplied through a bslot
+      const int e = tid + kGramThreads * p;
+      const int r = e % ROWS, c = e / ROWS;
+      const int slot = c < kl ? c : 16 * MT + (c - kl);
+      if (pb[p] == nullptr) {
+        tile[r * LD + slot] = va[p];
+      } else {
+        const ColDesc &dc = descs[c];
+        double2 o;
+        o.x = va[p].x - (dc.sig_re * vb[p].x - dc.sig_im * vb[p].y);
+        o.y = va[p].y - (dc.sig_re * vb[p].y + dc.sig_im * vb[p].x);
+        tile[r * LD + slot] = o;
+        if (dc.bslot >= 0) tile[r * LD + dc.bslot] = vb[p];
+      }
+    }
+  };
+  if (r0 < r1) fetch(r0);
+  for (int64_t rb = r0; rb < r1; rb += ROWS) {
+    store();
+    __syncthreads();
+    if (rb + ROWS < r1) fetch(rb + ROWS);
+#pragma unroll 2
+    for (int k0 = 0; k0 < ROWS; k0 += 4) {
+      const double2 *trow = tile + (k0 + lk) * LD;
+#pragma unroll
+      for (int q = 0; q < QMAX; ++q) {
+        if (!tv[q]) continue;
+        const double2 av = trow[16 * tm[q] + li];
+        const int jj = 16 * tn[q] + li;
+        const double2 bv = trow[jj < share ? jj : 16 * MT + (jj - share)];
+        accr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, accr[q], 0, 0, 0);
+        accr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.y, accr[q], 0, 0, 0);
+        acci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, acci[q], 0, 0, 0);
+        acci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.y, bv.x, acci[q], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  double2 *out = partial + size_t(blockIdx.x) * kl * kr;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    if (!tv[q]) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * tm[q] + 4 * v + lk, j = 16 * tn[q] + li;
+      if (i < kl && j < kr) out[i + size_t(kl) * j] = make_double2(accr[q][v], acci[q][v]);
+    }
+  }
 }
 
 // Sum of nparts partial results per element: one wavefront per element, each
@@ -885,7 +1031,7 @@ namespace {
 template <int MA, int NB>
 void gram_launch(const ColDesc *descs, int kl, int kr, int share, int64_t d, int64_t rpb,
                  double2 *partial, hipStream_t s) {
-  hipLaunchKernelGGL((k_gram<MA, NB>), dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr,
+  hipLaunchKernelGGL((k_gram_mfma<MA, NB>), dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr,
                      share, d, rpb, partial);
 }
 
@@ -910,7 +1056,7 @@ hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t 
   if (kl < 1 || kr < 1 || kl > kGramMaxL || kr > kGramMaxR) return hipErrorInvalidValue;
   if (share < 0 || share > kl || share > kr) return hipErrorInvalidValue;
   int64_t rpb = (d + kGramBlocks - 1) / kGramBlocks;
-  rpb = (rpb + kGramRows - 1) / kGramRows * kGramRows;
+  rpb = (rpb + 63) / 64 * 64;  // a whole number of tiles for every tile height
   const int ma = (kl + 15) / 16, nb = (kr + 15) / 16;
   double2 *pp = reinterpret_cast<double2 *>(partial);
   switch (ma) {

@@ -389,6 +389,57 @@ static int select_eigen(int n, const cd *a, int lda, int p, Key key, std::vector
 }
 // LU with partial pivoting, solve A X = B in place (B n x m).  Returns 0 or
 // the 1-based index of a zero pivot.
+// Least squares min ||A Y - B||_F for a full-column-rank A (rows >= cols),
+// by Householder QR (Golub & Van Loan 5.2.1, 5.3.3).  A (rows x cols, lda)
+// and B (rows x nrhs, ldb) are overwritten; Y is returned in the leading cols
+// rows of B.  Returns 0, or j + 1 when column j of the triangular factor is
+// numerically zero.
+int zlstsq(int rows, int cols, cd *a, int lda, cd *b, int ldb, int nrhs) {
+  if (rows < cols) return -1;
+  double amax = 0.0;
+  for (int j = 0; j < cols; ++j)
+    for (int i = 0; i < rows; ++i) amax = std::max(amax, std::abs(a[i + size_t(lda) * j]));
+  std::vector<cd> v(rows);
+  for (int j = 0; j < cols; ++j) {
+    double nrm2 = 0.0;
+    for (int i = j; i < rows; ++i) nrm2 += std::norm(a[i + size_t(lda) * j]);
+    const double nrm = std::sqrt(nrm2);
+    if (nrm == 0.0) return j + 1;
+    const cd x0 = a[j + size_t(lda) * j];
+    const double ax0 = std::abs(x0);
+    const cd ph = ax0 > 0.0 ? x0 / ax0 : cd(1.0, 0.0);
+    const cd alpha = -ph * nrm;  // reflect x onto alpha e_1, |alpha| = ||x||
+    for (int i = j; i < rows; ++i) v[i] = a[i + size_t(lda) * j];
+    v[j] -= alpha;
+    double vn2 = 0.0;
+    for (int i = j; i < rows; ++i) vn2 += std::norm(v[i]);
+    if (vn2 > 0.0) {
+      // apply I - 2 v v^H / (v^H v) to the trailing columns of A and to B
+      auto reflect = [&](cd *col) {
+        cd dot = 0.0;
+        for (int i = j; i < rows; ++i) dot += std::conj(v[i]) * col[i];
+        const cd f = 2.0 * dot / vn2;
+        for (int i = j; i < rows; ++i) col[i] -= f * v[i];
+      };
+      for (int jj = j + 1; jj < cols; ++jj) reflect(a + size_t(lda) * jj);
+      for (int r = 0; r < nrhs; ++r) reflect(b + size_t(ldb) * r);
+    }
+    a[j + size_t(lda) * j] = alpha;
+    for (int i = j + 1; i < rows; ++i) a[i + size_t(lda) * j] = 0.0;
+  }
+  for (int j = 0; j < cols; ++j)
+    if (!(std::abs(a[j + size_t(lda) * j]) > 1e-14 * amax)) return j + 1;
+  for (int r = 0; r < nrhs; ++r) {
+    cd *y = b + size_t(ldb) * r;
+    for (int i = cols - 1; i >= 0; --i) {
+      cd acc = y[i];
+      for (int jj = i + 1; jj < cols; ++jj) acc -= a[i + size_t(lda) * jj] * y[jj];
+      y[i] = acc / a[i + size_t(lda) * i];
+    }
+  }
+  return 0;
+}
+
 static int zgesv(int n, std::vector<cd> a, int m, cd *b, int ldb) {
   std::vector<int> piv(n);
   for (int k = 0; k < n; ++k) {

@@ -170,6 +170,20 @@ class HeliumGrid:
         check(self._lib.dex_get_kernel_timing(self._ctx, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def kernel_stats(self) -> dict:
+        """{class: (device ms, launches, algorithmic bytes, algorithmic flops)}
+        accumulated while timing is on."""
+        out = {}
+        for i, name in enumerate(_lib.KERNEL_CLASSES):
+            ms, n, by, fl = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
+            check(self._lib.dex_get_kernel_stats(self._ctx, i, C.byref(ms), C.byref(n), C.byref(by),
+                                                 C.byref(fl)))
+            out[name] = (ms.value, n.value, by.value, fl.value)
+        return out
+
+    def reset_kernel_stats(self) -> None:
+        check(self._lib.dex_reset_kernel_stats(self._ctx))
+
     def residuals_device(self, x, e: np.ndarray) -> np.ndarray:
         nev = e.size
         ee = np.ascontiguousarray(e, dtype=np.complex128)

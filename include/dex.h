@@ -211,6 +211,26 @@ const char *dex_last_error(void);
 int dex_set_kernel_timing(dex_ctx *ctx, int32_t enable);
 int dex_get_kernel_timing(const dex_ctx *ctx, double *pass1_ms, double *pass2_ms);
 
+/* With timing enabled, every launch of the library's kernels is bracketed by
+ * HIP events on the context's stream (resolved lazily, so no synchronisation
+ * is added to the work); per kernel class the summed device time, the number
+ * of launches and the launches' algorithmic HBM bytes and flops (counted on
+ * the host from the launch shapes) accumulate until dex_reset_kernel_stats.  No reference
+ * counterpart (the reference times with MPI_WTIME only, gplmr.F90:64-380). */
+enum {
+  DEX_K_PASS1 = 0,   /* electron-1 three-axis kinetic pass */
+  DEX_K_PASS2 = 1,   /* electron-2 pass with the potential (general vectors) */
+  DEX_K_SYMADD = 2,  /* exchange transpose-add pass with the potential */
+  DEX_K_GRAM = 3,    /* block Gram (with its partial-sum reduction) */
+  DEX_K_COMBINE = 4, /* block linear combination */
+  DEX_K_COLDOTS = 5, /* per-column dots and norms */
+  DEX_K_GENERIC = 6, /* generic-grid matvec */
+  DEX_KCLASSES = 7
+};
+int dex_get_kernel_stats(dex_ctx *ctx, int32_t kclass, double *total_ms, int64_t *launches,
+                         double *alg_bytes, double *alg_flops);
+int dex_reset_kernel_stats(dex_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif
