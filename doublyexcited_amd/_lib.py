@@ -65,7 +65,7 @@ class DexSolveOpts(C.Structure):
     _fields_ = [("strict_cholesky", C.c_int32), ("verbose", C.c_int32),
                 ("refresh_every", C.c_int32), ("fp64_restarts", C.c_int32),
                 ("fp32_tol", C.c_double), ("symmetry", C.c_int32), ("lock", C.c_int32),
-                ("inner_block", C.c_int32)]
+                ("inner_block", C.c_int32), ("guard", C.c_int32)]
 
 
 class DexError(RuntimeError):
@@ -108,7 +108,7 @@ def load() -> C.CDLL:
         "dex_fill_gaussian_device": (C.c_int, [P, P, C.c_int32, C.c_uint64, C.c_uint32, C.c_int32]),
         "dex_sync": (C.c_int, [P]),
         "dex_residuals_device": (C.c_int, [P, P, dp, C.c_int32, dp]),
-        "dex_gplmr_footprint": (C.c_int64, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+        "dex_gplmr_footprint": (C.c_int64, [P, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
         "dex_precblock_device": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                            C.c_double, C.c_double, C.c_int32, C.c_int32]),
         "dex_status_string": (C.c_char_p, [C.c_int]),

@@ -918,16 +918,18 @@ struct Solver {
   // Block classical Gram-Schmidt with cholQR in one pass: w <- (w - q C) R^{-1}
   // with C = q^H w and R^H R = w^H w - C^H C, i.e. one Gram [w; q]^H w and one
   // combine over [w q].  A second pass runs when the projection removed more
-  // than half of some column's norm squared (the criterion of Daniel, Gragg,
-  // Kaufman and Stewart, Math. Comp. 30 (1976) 772-795).  q must be
-  // orthonormal.  The input is `wd` (descriptors: plain columns or a - sigma b),
+  // than the fraction 1 - keep_min of some column's norm squared: keep_min =
+  // 0.5 is the criterion of Daniel, Gragg, Kaufman and Stewart (Math. Comp. 30
+  // (1976) 772-795); the subtraction loses about eps / keep of relative
+  // accuracy in R, so a smaller keep_min trades orthogonality for a pass.  q
+  // must be orthonormal.  The input is `wd` (descriptors: plain columns or a - sigma b),
   // the result goes to `w` (which may be the storage wd reads).  On success
   // `cq` and `r` satisfy w_in = q cq + w_out r; `ok` is false when the
   // projected block is numerically rank deficient (then w holds the projected,
   // unnormalised block and the caller decides).  `materialize` false skips the
   // final combine (the Arnoldi step that only needs C and R).
   int cgs_cholqr(const Block &q, const std::vector<ColDesc> &wd, Block &w, std::vector<cd> &cq,
-                 std::vector<cd> &r, bool &ok, bool materialize = true) {
+                 std::vector<cd> &r, bool &ok, bool materialize = true, double keep_min = 0.5) {
     const int k = int(wd.size()), kq = q.size();
     std::vector<cd> g, p, rinv;
     double keep = 1.0;
@@ -939,7 +941,7 @@ struct Solver {
     cq.assign(size_t(kq) * k, 0.0);
     for (int j = 0; j < k; ++j)
       for (int i = 0; i < kq; ++i) cq[i + size_t(kq) * j] = g[k + i + size_t(k + kq) * j];
-    bool good = keep > 0.5 && chol_inverse(p, w2, k, r, rinv);
+    bool good = keep > keep_min && chol_inverse(p, w2, k, r, rinv);
     t_small += now_s() - t0;
     if (good) {
       ok = true;
@@ -1072,7 +1074,9 @@ struct Solver {
       for (int i = 0; i < kb; ++i) wd.push_back(shifted(hq.cols[i], qj.cols[i], sigma));
       std::vector<cd> cj, rj;
       const bool last = j == m - 1;
-      if (int st = cgs_cholqr(qall, wd, hq, cj, rj, ok, !last)) return st;
+      // a preconditioner's basis: one pass unless the projection cancels
+      // more than 99.9% of a column (R keeps ~1e-13 relative accuracy)
+      if (int st = cgs_cholqr(qall, wd, hq, cj, rj, ok, !last, 1e-3)) return st;
       if (!ok) rj.assign(size_t(kb) * kb, 0.0);  // (H - sigma) Q_j lies in span(Q): the space is exhausted
       hc.push_back(cj);
       hr.push_back(rj);
@@ -1158,15 +1162,19 @@ struct Solver {
   }
 };
 
-int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int maxit, double tol,
-              int inner, int outmodulus, const dex_solve_opts &o, dex_result *res, bool single,
-              std::vector<Dbl2 *> &store) {
+// One GPLMR run on the block `xcols` (the caller's columns, then any guard
+// columns); convergence and the returned values concern the first `nconv`
+// columns, which the harmonic-Ritz selection keeps ordered nearest-first.
+int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out, cd sigma, int ns,
+              int maxit, double tol, int inner, int outmodulus, const dex_solve_opts &o, dex_result *res,
+              bool single, std::vector<Dbl2 *> &store) {
+  const int nev = int(xcols.size());
   Solver S{c, nev, ns, o.strict_cholesky != 0, single, o.symmetry};
   // every buffer in `store` is free at entry
   const size_t vbytes = size_t(c->d) * sizeof(Dbl2);
-  // columns: X in xdev; the rest from `store` (pool of (5 + 2 ns) nev vectors)
+  // columns: X in xcols; the rest from `store` (pool of (5 + 2 ns) nev vectors)
   Block X, AX, W, AW, P, AP;
-  for (int j = 0; j < nev; ++j) X.cols.push_back(xdev + size_t(j) * c->d);
+  X.cols = xcols;
   S.pool = store;
   S.store = &store;
   if (int st = S.grow(AX, nev)) return st;
@@ -1245,7 +1253,7 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
   };
   // First E and W: X is already unit, the scaling is 1.
   if (int st = residual()) return st;
-  double maxres = *std::max_element(resn.begin(), resn.end());
+  double maxres = *std::max_element(resn.begin(), resn.begin() + nconv);
   int iter = 1;
   Block Sblk, ASblk;
   while (iter <= maxit && maxres > tol) {
@@ -1394,7 +1402,7 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     // W gets nev columns again
     if (int st = S.grow(W, nev - W.size())) return st;
     if (int st = residual()) return st;
-    maxres = *std::max_element(resn.begin(), resn.end());
+    maxres = *std::max_element(resn.begin(), resn.begin() + nconv);
     if (o.verbose && (maxres <= tol || (outmodulus > 0 && iter % outmodulus == 0))) {
       std::printf("ITERATION %5d\n", iter);
       for (int j = 0; j < nev; ++j)
@@ -1403,7 +1411,7 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
     }
     ++iter;
   }
-  for (int j = 0; j < nev; ++j) e_out[j] = e[j];
+  for (int j = 0; j < nconv; ++j) e_out[j] = e[j];
   if (res) {
     res->iterations = iter - 1;
     res->converged = maxres <= tol;
@@ -1423,15 +1431,19 @@ int gplmr_run(dex_ctx *c, Dbl2 *xdev, cd *e_out, cd sigma, int nev, int ns, int 
 }  // namespace
 
 extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns, int32_t inner_iter,
-                                       int32_t inner_block) {
+                                       const dex_solve_opts *opts) {
   if (!c) return -1;
-  // The caller's block X (NEV), the solver's (6 + 2 NS) NEV work vectors, the
-  // matvec intermediate U, and for an inner solve of order m on groups of g
-  // columns its 2 m g basis vectors (V and B) plus 2 g for the next Krylov block.
+  // The caller's block X (NEV), the guard columns, the solver's (6 + 2 NS)
+  // (NEV + guard) work vectors, the matvec intermediate U, and for an inner
+  // solve of order m on groups of g columns its Arnoldi basis ((m + 1) g) and
+  // the next block (g).
   const int64_t vec = c->d * 16;
-  const int64_t grp = inner_block > 0 && inner_block < nev ? inner_block : nev;
-  const int64_t inner = inner_iter >= 2 ? int64_t(2 * inner_iter + 2) * grp : (inner_iter == 1 ? nev : 0);
-  return (int64_t(7 + 2 * ns) * nev + inner) * vec + vec;
+  const int64_t guard = opts && opts->guard > 0 ? opts->guard : 0;
+  const int64_t nblk = nev + guard;
+  const int64_t ib = opts ? opts->inner_block : 0;
+  const int64_t grp = ib > 0 && ib < nblk ? ib : nblk;
+  const int64_t inner = inner_iter >= 2 ? int64_t(inner_iter + 2) * grp : (inner_iter == 1 ? nblk : 0);
+  return (int64_t(7 + 2 * ns) * nblk + inner) * vec + vec;
 }
 
 extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,
@@ -1440,7 +1452,10 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
                                     const dex_solve_opts *opts, dex_result *res) {
   if (!c || !x_dev || !e) return fail(DEX_EINVAL, "null argument");
   if (int64_t(m) * mb != c->d) return fail(DEX_ESIZE, "SIZE ERROR: M*MB != NN^6");
-  if (nev < 1 || ns < 1 || (3 + ns) * nev > 64) return fail(DEX_EINVAL, "need NEV >= 1, NS >= 1, (3+NS)*NEV <= 64");
+  const int guard = opts ? opts->guard : 0;
+  if (guard < 0) return fail(DEX_EINVAL, "guard must be >= 0");
+  if (nev < 1 || ns < 1 || (3 + ns) * (nev + guard) > 64)
+    return fail(DEX_EINVAL, "need NEV >= 1, NS >= 1, (3+NS)*(NEV+guard) <= 64");
   if (inner_iter < 0) return fail(DEX_EINVAL, "GM_MAX_INNER_ITER must be >= 0");
   if (opts && (opts->symmetry < -1 || opts->symmetry > 1)) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
   if (opts && opts->symmetry != 0 && !c->fast64 && !(c->p.precision == DEX_FP32 && c->fast32))
@@ -1450,16 +1465,27 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   dex_solve_opts_default(&o);
   if (opts) o = *opts;
   if (res) std::memset(res, 0, sizeof(*res));
-  const int nwork = (5 + 2 * ns) * nev + nev;  // pool plus headroom for the new P block
-  std::vector<Dbl2 *> store;
-  for (int j = 0; j < nwork; ++j) {
+  const int nblk = nev + guard;
+  const int nwork = (5 + 2 * ns) * nblk + nblk;  // pool plus headroom for the new P block
+  std::vector<Dbl2 *> store, guards;
+  for (int j = 0; j < nwork + guard; ++j) {
     Dbl2 *p = nullptr;
     hipError_t err = hipMalloc(&p, size_t(c->d) * sizeof(Dbl2));
     if (err != hipSuccess) {
       for (Dbl2 *q : store) (void)hipFree(q);
+      for (Dbl2 *q : guards) (void)hipFree(q);
       return fail(DEX_EOOM, "device allocation for GPLMR work vectors failed");
     }
-    store.push_back(p);
+    (j < nwork ? store : guards).push_back(p);
+  }
+  // the block: the caller's NEV columns, then the guard columns (seeded
+  // complex Gaussians with the start block's exchange symmetry)
+  std::vector<Dbl2 *> xcols;
+  for (int j = 0; j < nev; ++j) xcols.push_back(static_cast<Dbl2 *>(x_dev) + size_t(j) * c->d);
+  for (int j = 0; j < guard; ++j) {
+    DEX_HIP(dex::launch_fill_gaussian(guards[j], c->d, 0x67756172ull, uint32_t(j), 0xFFFFFFFFu, c->stream));
+    if (o.symmetry != 0) DEX_HIP(dex::launch_sym_project(guards[j], c->m, o.symmetry, c->stream));
+    xcols.push_back(guards[j]);
   }
   const cd sigma(sigma_re, sigma_im);
   std::vector<cd> ev(nev);
@@ -1468,13 +1494,13 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   // FP32 contexts: the FP32-matvec phase runs to max(tol, fp32_tol); FP64
   // restarts from its block then reach tol.
   const double tol1 = mixed ? std::max(tol, o.fp32_tol) : tol;
-  st = gplmr_run(c, static_cast<Dbl2 *>(x_dev), ev.data(), sigma, nev, ns, maxit, tol1, inner_iter,
-                 outmodulus, o, res, mixed, store);
+  st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol1, inner_iter, outmodulus, o, res, mixed,
+                 store);
   if (mixed && (st == DEX_OK || st == DEX_ENOTCONVERGED)) {
     for (int r = 0; r < o.fp64_restarts; ++r) {
       dex_result sub{};
-      st = gplmr_run(c, static_cast<Dbl2 *>(x_dev), ev.data(), sigma, nev, ns, maxit, tol, inner_iter,
-                     outmodulus, o, res ? &sub : nullptr, false, store);
+      st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol, inner_iter, outmodulus, o,
+                     res ? &sub : nullptr, false, store);
       if (res) {
         res->matvecs += sub.matvecs;
         res->iterations += sub.iterations;
@@ -1493,6 +1519,7 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
     }
   }
   for (Dbl2 *q : store) (void)hipFree(q);
+  for (Dbl2 *q : guards) (void)hipFree(q);
   for (int j = 0; j < nev; ++j) {
     e[2 * j] = ev[j].real();
     e[2 * j + 1] = ev[j].imag();

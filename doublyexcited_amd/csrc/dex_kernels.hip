@@ -840,6 +840,9 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
                                                           const double2 *__restrict__ beta, int64_t d) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NP = kCombGroups * JPT;
+  // input elements per thread per tile: all issued before any wait, the next
+  // tile's while this one is combined
+  constexpr int PMAX = kCombRows * kCombMaxIn / kCombThreads;
   double2 *sc = reinterpret_cast<double2 *>(smem);  // [nin][NP]
   double2 *tin = sc + size_t(nin) * NP;              // [nin][kCombRows]
   const int tid = threadIdx.x;
@@ -849,16 +852,53 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
     const int i = e / NP, j = e % NP;
     sc[e] = j < nout ? cm[i + size_t(nin) * j] : make_double2(0.0, 0.0);
   }
+  // element p of this thread: row e % kCombRows of input column e / kCombRows
+  const double2 *pa[PMAX];
+  const double2 *pb[PMAX];
+#pragma unroll
+  for (int p = 0; p < PMAX; ++p) {
+    const int e = tid + kCombThreads * p;
+    pa[p] = nullptr;
+    pb[p] = nullptr;
+    if (e < kCombRows * nin) {
+      const ColDesc &dc = descs[e / kCombRows];
+      pa[p] = reinterpret_cast<const double2 *>(dc.a);
+      pb[p] = reinterpret_cast<const double2 *>(dc.b);
+    }
+  }
+  double2 va[PMAX], vb[PMAX];
+  auto fetch = [&](int64_t rb) {
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+      const int64_t r = rb + (tid + kCombThreads * p) % kCombRows;
+      if (pa[p] != nullptr && r < d) {
+        va[p] = pa[p][r];
+        if (pb[p] != nullptr) vb[p] = pb[p][r];
+      }
+    }
+  };
   const int64_t ntile = (d + kCombRows - 1) / kCombRows;
+  if (int64_t(blockIdx.x) < ntile) fetch(int64_t(blockIdx.x) * kCombRows);
   for (int64_t tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
     const int64_t rb = tb * kCombRows;
     const int nr = int(d - rb < kCombRows ? d - rb : kCombRows);
     __syncthreads();
-    for (int e = tid; e < kCombRows * nin; e += kCombThreads) {
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+      if (pa[p] == nullptr) continue;
+      const int e = tid + kCombThreads * p;
       const int r = e % kCombRows, c = e / kCombRows;
-      if (r < nr) tin[c * kCombRows + r] = col_value(descs[c], rb + r);
+      if (r >= nr) continue;
+      double2 v = va[p];
+      if (pb[p] != nullptr) {
+        const ColDesc &dc = descs[c];
+        v.x -= dc.sig_re * vb[p].x - dc.sig_im * vb[p].y;
+        v.y -= dc.sig_re * vb[p].y + dc.sig_im * vb[p].x;
+      }
+      tin[c * kCombRows + r] = v;
     }
     __syncthreads();
+    if (tb + gridDim.x < ntile) fetch((tb + gridDim.x) * kCombRows);
     if (row < nr) {
       double ax[JPT], ay[JPT];
 #pragma unroll
@@ -890,6 +930,66 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
           }
           *o = make_double2(ox, oy);
         }
+      }
+    }
+  }
+}
+
+// The combine for narrow outputs (nout <= 16), without LDS: a wavefront owns
+// 64 consecutive rows (one per lane) and every output column of them, so the
+// coefficients C(i, j) are wavefront-uniform (scalar loads, no LDS traffic)
+// and each input element is read once, by one lane, in one coalesced 1 KB
+// row segment per column.  All inputs of a row are read before any output of
+// it is written, so outputs may alias inputs.
+template <int JN>
+__global__ __launch_bounds__(256) void k_combine_w(const ColDesc *__restrict__ descs, int nin, int nout,
+                                                   const double2 *__restrict__ cm,
+                                                   const double2 *__restrict__ beta, int64_t d) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = int64_t(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t stride = int64_t(gridDim.x) * 4 * 64;
+  for (int64_t rb = wave * 64; rb < d; rb += stride) {
+    const int64_t r = rb + lane;
+    const bool in_range = r < d;
+    double ax[JN], ay[JN];
+#pragma unroll
+    for (int q = 0; q < JN; ++q) ax[q] = ay[q] = 0.0;
+#pragma unroll 4
+    for (int i = 0; i < nin; ++i) {
+      const ColDesc dc = descs[i];
+      double2 v = make_double2(0.0, 0.0);
+      if (in_range) {
+        v = reinterpret_cast<const double2 *>(dc.a)[r];
+        if (dc.b != nullptr) {
+          const double2 vb = reinterpret_cast<const double2 *>(dc.b)[r];
+          v.x -= dc.sig_re * vb.x - dc.sig_im * vb.y;
+          v.y -= dc.sig_re * vb.y + dc.sig_im * vb.x;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < JN; ++q) {
+        if (q < nout) {
+          const double2 c = cm[i + size_t(nin) * q];
+          ax[q] = fma(v.x, c.x, ax[q]);
+          ax[q] = fma(-v.y, c.y, ax[q]);
+          ay[q] = fma(v.x, c.y, ay[q]);
+          ay[q] = fma(v.y, c.x, ay[q]);
+        }
+      }
+    }
+    if (!in_range) continue;
+#pragma unroll
+    for (int q = 0; q < JN; ++q) {
+      if (q < nout) {
+        double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + q].a)) + r;
+        double ox = ax[q], oy = ay[q];
+        if (beta != nullptr) {
+          const double2 bq = beta[q];
+          const double2 ov = *o;
+          ox += bq.x * ov.x - bq.y * ov.y;
+          oy += bq.x * ov.y + bq.y * ov.x;
+        }
+        *o = make_double2(ox, oy);
       }
     }
   }
@@ -1095,6 +1195,19 @@ hipError_t combine_launch(const ColDesc *descs, int nin, int nout, const Dbl2 *c
 hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
                           int64_t d, hipStream_t s) {
   if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombGroups * 6) return hipErrorInvalidValue;
+  if (nout <= 16) {
+    const int64_t waves = (d + 63) / 64;
+    const unsigned grid = unsigned((waves + 3) / 4 < 2048 ? (waves + 3) / 4 : 2048);
+    const double2 *cc = reinterpret_cast<const double2 *>(c);
+    const double2 *bb = reinterpret_cast<const double2 *>(beta);
+    if (nout <= 4)
+      hipLaunchKernelGGL(k_combine_w<4>, dim3(grid), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
+    else if (nout <= 8)
+      hipLaunchKernelGGL(k_combine_w<8>, dim3(grid), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
+    else
+      hipLaunchKernelGGL(k_combine_w<16>, dim3(grid), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
+    return hipGetLastError();
+  }
   switch ((nout + kCombGroups - 1) / kCombGroups) {
     case 1: return combine_launch<1>(descs, nin, nout, c, beta, d, s);
     case 2: return combine_launch<2>(descs, nin, nout, c, beta, d, s);

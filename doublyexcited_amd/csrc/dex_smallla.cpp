@@ -387,8 +387,6 @@ static int select_eigen(int n, const cd *a, int lda, int p, Key key, std::vector
   leading_eigvecs(n, p, t, z, anorm, vec);
   return 0;
 }
-// LU with partial pivoting, solve A X = B in place (B n x m).  Returns 0 or
-// the 1-based index of a zero pivot.
 // Least squares min ||A Y - B||_F for a full-column-rank A (rows >= cols),
 // by Householder QR (Golub & Van Loan 5.2.1, 5.3.3).  A (rows x cols, lda)
 // and B (rows x nrhs, ldb) are overwritten; Y is returned in the leading cols
@@ -440,6 +438,8 @@ int zlstsq(int rows, int cols, cd *a, int lda, cd *b, int ldb, int nrhs) {
   return 0;
 }
 
+// LU with partial pivoting, solve A X = B in place (B n x m).  Returns 0 or
+// the 1-based index of a zero pivot.
 static int zgesv(int n, std::vector<cd> a, int m, cd *b, int ldb) {
   std::vector<int> piv(n);
   for (int k = 0; k < n; ++k) {

@@ -35,6 +35,9 @@ class Request:
     precision: str = dataclasses.field(compare=False, default="fp64")
     maxit: int = dataclasses.field(compare=False, default=3000)
     seed: int = dataclasses.field(compare=False, default=0)
+    inner_iter: int = dataclasses.field(compare=False, default=0)   # GM_MAX_INNER_ITER
+    inner_block: int = dataclasses.field(compare=False, default=0)
+    guard: int = dataclasses.field(compare=False, default=0)
 
 
 @dataclasses.dataclass
@@ -59,13 +62,17 @@ def snap_nn(nn: int) -> int:
     return (nn - 1) // 2
 
 
-def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1, inner_iter: int = 0) -> int:
-    """Device bytes a solve holds: the block X, (6 + 2 NS) NEV work vectors, the
-    matvec intermediate (the same for both precisions: the basis is complex128)
-    and, for an inner solve of order m >= 2, its (2 m + 2) NEV basis vectors."""
+def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1, inner_iter: int = 0,
+                    inner_block: int = 0, guard: int = 0) -> int:
+    """Device bytes a solve holds (the library's own count): the block X, the
+    guard columns, (6 + 2 NS) (NEV + guard) work vectors, the matvec
+    intermediate (the same for both precisions: the basis is complex128) and,
+    for an inner solve of order m >= 2 on groups of g columns, (m + 2) g more."""
     d = (2 * n + 1) ** 6
-    inner = (2 * inner_iter + 2) * nev if inner_iter >= 2 else (nev if inner_iter == 1 else 0)
-    return ((7 + 2 * ns) * nev + inner) * 16 * d + 16 * d
+    nblk = nev + guard
+    grp = inner_block if 0 < inner_block < nblk else nblk
+    inner = (inner_iter + 2) * grp if inner_iter >= 2 else (nblk if inner_iter == 1 else 0)
+    return ((7 + 2 * ns) * nblk + inner) * 16 * d + 16 * d
 
 
 def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),
@@ -110,7 +117,8 @@ class Server:
                 x = torch.empty(g.d * req.nev, dtype=torch.complex128, device=f"cuda:{self.device}")
                 g.fill_gaussian_device(x, req.nev, seed=req.seed, symmetry=1)
                 r = g.gplmr_eig_device(x, req.sigma, req.nev, ns=1, maxit=req.maxit, tol=req.tol,
-                                       symmetry=1, fp32_tol=max(req.tol, 2e-6))
+                                       symmetry=1, fp32_tol=max(req.tol, 2e-6), inner_iter=req.inner_iter,
+                                       inner_block=req.inner_block, guard=req.guard)
                 res = g.residuals_device(x, r.e)
                 del x
             done = time.time() - t0
@@ -140,7 +148,8 @@ class Server:
             with self._cv:
                 if waiting and self._free_slots > 0:
                     _, _, req = waiting[0]
-                    need = footprint_bytes(req.n, req.nev, req.precision)
+                    need = footprint_bytes(req.n, req.nev, req.precision, 1, req.inner_iter,
+                                           req.inner_block, req.guard)
                     if need > self.cap:
                         heapq.heappop(waiting)
                         results[req.rid] = Result(req.rid, False, None, None, 0, 0, 0.0, 0.0,

@@ -136,8 +136,13 @@ typedef struct dex_solve_opts {
                               every column, as the reference */
   int32_t inner_block;     /* inner solve (inner_iter >= 2): >0 solves column groups of
                               this many at a time, each in its own block Krylov space
-                              (2 m inner_block basis vectors instead of 2 m NEV); 0
-                              (default): the whole block at once, as the reference */
+                              (an (m + 1) inner_block column basis instead of (m + 1) NEV);
+                              0 (default): the whole block at once, as the reference */
+  int32_t guard;           /* extra block columns (seeded Gaussians) iterated beside the
+                              caller's NEV: convergence is tested, and values returned, on
+                              the NEV nearest; a block that would end inside a degenerate
+                              level then holds the whole level.  0 (default): the
+                              reference's block of exactly NEV.  Needs (3+NS)(NEV+guard) <= 64 */
 } dex_solve_opts;
 
 void dex_solve_opts_default(dex_solve_opts *o);
@@ -185,10 +190,10 @@ int dex_precblock_device(dex_ctx *ctx, const void *x_dev, void *px_dev, int32_t 
                          int32_t symmetry);
 
 /* Bytes of device memory a dex_gplmr_eig_device solve of (nev, ns, inner_iter,
- * inner_block as in dex_solve_opts) holds on this context, the caller's block X
- * included. */
+ * opts: inner_block and guard; NULL for the defaults) holds on this context,
+ * the caller's block X included. */
 int64_t dex_gplmr_footprint(const dex_ctx *ctx, int32_t nev, int32_t ns, int32_t inner_iter,
-                            int32_t inner_block);
+                            const dex_solve_opts *opts);
 
 /* Host-only set-up and small-matrix routines (no device use), exported so the
  * host arithmetic can be checked on machines without a GPU:
