@@ -182,9 +182,9 @@ def test_mixed_precision_within_1e7(dex):
     assert match(r32.e, r64.e) <= 1e-7
 
 
-@pytest.mark.skip(reason="needs the reference's inner solve (GM_MAX_INNER_ITER 5-20 in its logged "
-                         "run; PRECBLOCK_BASIC is not built): without it GPLMR stalls near residual "
-                         "0.08 on this resonance at NEV=1, NS=1 and NS=5 alike")
+@pytest.mark.skip(reason="needs the reference's He+-orbital-product start vectors (DIAGSUBSPACE, "
+                         "hammod.F90:1130-1256): from a Gaussian singlet start the solver converges to "
+                         "another eigenpair (-0.638873+0.029806i) with or without the inner solve")
 def test_reference_logged_singlet_resonance(dex, golden_dir):
     """NN=11, delta=2, theta=0.2: the reference printed E = -0.713154142874 +
     0.035227459486 i at residual 7.99e-8 (Out.S.N5:147).  A singlet start with
