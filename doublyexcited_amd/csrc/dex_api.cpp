@@ -1561,6 +1561,173 @@ extern "C" int dex_precblock_device(dex_ctx *c, const void *x_dev, void *px_dev,
   return st;
 }
 
+// HELIUM_ORBITALGATHER (hammod.F90:1080-1113) orthonormalisation, then
+// DIAGSUBSPACE (hammod.F90:1125-1256) and GET_EXPECTS (hammod.F90:1261-1301).
+extern "C" int dex_diagsubspace_device(dex_ctx *c, const double *orbitals, int32_t numorbs, int32_t singlet,
+                                       int32_t hermopt, double etarget_re, double etarget_im, int32_t numout,
+                                       void *psi_dev, double *e0, double *expects) {
+  if (!c || !orbitals || !psi_dev || !e0 || !expects) return fail(DEX_EINVAL, "null argument");
+  if (numorbs < 1 || numorbs > 32) return fail(DEX_EINVAL, "numorbs must be 1..32");
+  const int k = numorbs;
+  const int64_t m = c->m;
+  const int sgn = singlet ? 1 : -1;
+  // the orbital-product basis: singlet a <= b, triplet a < b (:1170-1180)
+  std::vector<std::pair<int, int>> pairs;
+  for (int a = 0; a < k; ++a)
+    for (int b = a + (singlet ? 0 : 1); b < k; ++b) pairs.emplace_back(a, b);
+  const int nb = int(pairs.size());
+  if (numout < 1 || numout > nb) return fail(DEX_EINVAL, "numout must be 1..(size of the orbital-product basis)");
+  if (!(c->fast64)) return fail(DEX_EUNSUPPORTED, "needs the LDS matvec kernels for this grid");
+  DEX_HIP(hipSetDevice(c->p.device));
+  // orthonormalise the orbitals: bilinear (hermopt = 0) or Hermitian products,
+  // one Gram-Schmidt pass as HELIUM_ORBITALGATHER
+  std::vector<cd> x(size_t(m) * k);
+  for (size_t i = 0; i < x.size(); ++i) x[i] = cd(orbitals[2 * i], orbitals[2 * i + 1]);
+  auto dot = [&](int a, int b) {
+    cd sacc = 0.0;
+    for (int64_t p = 0; p < m; ++p) {
+      const cd xa = x[p + m * a];
+      sacc += (hermopt ? std::conj(xa) : xa) * x[p + m * b];
+    }
+    return sacc;
+  };
+  for (int i = 0; i < k; ++i) {
+    for (int j = 0; j < i; ++j) {
+      const cd pr = dot(j, i);
+      for (int64_t p = 0; p < m; ++p) x[p + m * i] -= x[p + m * j] * pr;
+    }
+    const cd nrm = std::sqrt(dot(i, i));
+    if (std::abs(nrm) == 0.0) return fail(DEX_EZEROBLOCK, "zero orbital");
+    for (int64_t p = 0; p < m; ++p) x[p + m * i] /= nrm;
+  }
+  // device buffers: X, conj-or-plain X for the Y X products, T, W, one vector
+  Dbl2 *dx = nullptr, *dt = nullptr, *dw = nullptr, *dpsi = nullptr, *dy = nullptr;
+  auto cleanup = [&]() {
+    for (void *ptr : {(void *)dx, (void *)dt, (void *)dw, (void *)dpsi, (void *)dy})
+      if (ptr) (void)hipFree(ptr);
+  };
+  const size_t mk = size_t(m) * k * sizeof(Dbl2);
+  if (hipMalloc(&dx, mk) != hipSuccess || hipMalloc(&dt, mk) != hipSuccess || hipMalloc(&dw, mk) != hipSuccess ||
+      hipMalloc(&dpsi, size_t(c->d) * sizeof(Dbl2)) != hipSuccess ||
+      hipMalloc(&dy, size_t(c->d) * sizeof(Dbl2)) != hipSuccess) {
+    cleanup();
+    return fail(DEX_EOOM, "device allocation for the orbital subspace failed");
+  }
+  int st = DEX_OK;
+  auto hip = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && st == DEX_OK) st = fail(DEX_EHIP, what);
+    return st == DEX_OK;
+  };
+  hip(hipMemcpyAsync(dx, x.data(), mk, hipMemcpyHostToDevice, c->stream), "upload orbitals");
+  // coefficient block S (k x k) of a combination sum_I c_I psi_I -> T = X S
+  auto build = [&](const std::vector<cd> &cvec, Dbl2 *out) {
+    std::vector<cd> sm(size_t(k) * k, 0.0);
+    for (int i = 0; i < nb; ++i) {
+      const int a = pairs[i].first, b = pairs[i].second;
+      const double n = a == b ? 0.5 : 1.0 / std::sqrt(2.0);
+      sm[a + size_t(k) * b] += cvec[i] * n;
+      sm[b + size_t(k) * a] += double(sgn) * cvec[i] * n;
+    }
+    std::vector<cd> t(size_t(m) * k, 0.0);
+    for (int b = 0; b < k; ++b)
+      for (int a = 0; a < k; ++a) {
+        const cd sab = sm[a + size_t(k) * b];
+        if (sab == cd(0.0)) continue;
+        for (int64_t p = 0; p < m; ++p) t[p + m * b] += x[p + m * a] * sab;
+      }
+    if (!hip(hipMemcpyAsync(dt, t.data(), mk, hipMemcpyHostToDevice, c->stream), "upload T")) return false;
+    if (!hip(dex::launch_lowrank(dt, dx, out, m, k, c->stream), "lowrank")) return false;
+    return hip(hipStreamSynchronize(c->stream), "sync");  // t is a host temporary
+  };
+  // A(I, J) = psi_I . H psi_J (bilinear) or psi_I^H H psi_J (:1196-1220)
+  std::vector<cd> amat(size_t(nb) * nb);
+  std::vector<cd> wh(size_t(m) * k);
+  for (int jj = 0; jj < nb && st == DEX_OK; ++jj) {
+    std::vector<cd> cvec(nb, 0.0);
+    cvec[jj] = 1.0;
+    if (!build(cvec, dpsi)) break;
+    if (int e = apply_h(c, dpsi, dy, false, sgn)) {
+      st = e;
+      break;
+    }
+    if (!hip(dex::launch_ywx(dy, dx, dw, m, k, hermopt ? 1 : 0, c->stream), "ywx")) break;
+    if (!hip(hipMemcpyAsync(wh.data(), dw, mk, hipMemcpyDeviceToHost, c->stream), "download W")) break;
+    if (!hip(hipStreamSynchronize(c->stream), "sync")) break;
+    // G(a, b) = x_a . W_b (conjugated x_a for the Hermitian product)
+    std::vector<cd> g(size_t(k) * k, 0.0);
+    for (int b = 0; b < k; ++b)
+      for (int a = 0; a < k; ++a) {
+        cd sacc = 0.0;
+        for (int64_t p = 0; p < m; ++p) {
+          const cd xa = x[p + m * a];
+          sacc += (hermopt ? std::conj(xa) : xa) * wh[p + m * b];
+        }
+        g[a + size_t(k) * b] = sacc;
+      }
+    for (int ii = 0; ii < nb; ++ii) {
+      const int a = pairs[ii].first, b = pairs[ii].second;
+      const double n = a == b ? 0.5 : 1.0 / std::sqrt(2.0);
+      amat[ii + size_t(nb) * jj] = n * (g[a + size_t(k) * b] + double(sgn) * g[b + size_t(k) * a]);
+    }
+  }
+  std::vector<cd> lam, vec;
+  if (st == DEX_OK && dex::zgeev_right(nb, amat.data(), nb, lam, vec) != 0)
+    st = fail(DEX_EGEVP, "subspace eigenproblem failed");
+  if (st == DEX_OK) {
+    // mysort (hammod.F90:1658-1690): ascending |E0 - etarget|, the last index on ties
+    const cd et(etarget_re, etarget_im);
+    std::vector<int> order, taken(nb, 0);
+    for (int j = 0; j < nb; ++j) {
+      int best = -1;
+      double low = 0.0;
+      for (int i = 0; i < nb; ++i) {
+        if (taken[i]) continue;
+        const double dv = std::abs(lam[i] - et);
+        if (best < 0 || dv <= low) {
+          best = i;
+          low = dv;
+        }
+      }
+      taken[best] = 1;
+      order.push_back(best);
+    }
+    for (int o = 0; o < numout && st == DEX_OK; ++o) {
+      const int col = order[o];
+      std::vector<cd> cvec(vec.begin() + size_t(nb) * col, vec.begin() + size_t(nb) * (col + 1));
+      Dbl2 *out = static_cast<Dbl2 *>(psi_dev) + size_t(o) * c->d;
+      if (!build(cvec, out)) break;
+      // Hermitian normalisation (:1250-1252) and GET_EXPECTS with the Hermitian product
+      std::vector<double> aa, bb;
+      std::vector<cd> ab;
+      if (int e = coldots(c, {plain(out)}, {}, aa, ab, bb)) {
+        st = e;
+        break;
+      }
+      std::vector<cd> sc(1, 1.0 / std::sqrt(aa[0]));
+      if (int e = combine(c, {plain(out)}, {out}, sc)) {
+        st = e;
+        break;
+      }
+      if (int e = apply_h(c, out, dy, false, sgn)) {
+        st = e;
+        break;
+      }
+      if (int e = coldots(c, {plain(out)}, {plain(dy)}, aa, ab, bb)) {
+        st = e;
+        break;
+      }
+      e0[2 * o] = lam[col].real();
+      e0[2 * o + 1] = lam[col].imag();
+      const cd ex = ab[0] / aa[0];
+      expects[2 * o] = ex.real();
+      expects[2 * o + 1] = ex.imag();
+    }
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && st == DEX_OK) st = fail(DEX_EHIP, "sync");
+  cleanup();
+  return st;
+}
+
 extern "C" int dex_gplmr_eig(dex_ctx *c, double *x, double *e, int32_t m, int32_t mb, double sigma_re,
                              double sigma_im, int32_t nev, int32_t ns, int32_t maxit, double tol,
                              int32_t inner_iter, int32_t outmodulus, const dex_solve_opts *opts,

@@ -101,6 +101,10 @@ hipError_t launch_fill_gaussian(Dbl2 *x, int64_t n, uint64_t seed, uint32_t col,
                                 hipStream_t s);
 // Exchange projection X <- (X + sign X^T)/2 in the M x M view (in place).
 hipError_t launch_sym_project(Dbl2 *x, int64_t m, int sign, hipStream_t s);
+// out(p, q) = sum_b T(p, b) X(q, b) (M x M from two M x K blocks)
+hipError_t launch_lowrank(const Dbl2 *t, const Dbl2 *x, Dbl2 *out, int64_t m, int k, hipStream_t s);
+// W = Y X (conj_x: Y conj(X)) for M x M Y and M x K X
+hipError_t launch_ywx(const Dbl2 *y, const Dbl2 *x, Dbl2 *w, int64_t m, int k, int conj_x, hipStream_t s);
 // Second matvec pass for X = sign X^T: Y = s2 (U + sign U^T) + c1 Vr o (X + sign X^T)/2.
 hipError_t launch_symadd(int nn, int io, const void *x, const void *u, void *y,
                          const double *vn, const double *tinv, const MatvecCoef &coef, int sign,

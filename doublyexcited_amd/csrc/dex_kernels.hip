@@ -1121,7 +1121,76 @@ __global__ void k_to_double(const float2 *__restrict__ x, double2 *__restrict__ 
   if (i < n) y[i] = make_double2(double(x[i].x), double(x[i].y));
 }
 
+// Two-electron vector of rank <= K over one-electron orbitals:
+// out(p, q) = sum_b T(p, b) X(q, b), p electron 1 (fastest), q electron 2;
+// T and X are M x K column-major.  One thread per element.
+__global__ __launch_bounds__(256) void k_lowrank(const double2 *__restrict__ t, const double2 *__restrict__ x,
+                                                 double2 *__restrict__ out, int64_t m, int k) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m * m) return;
+  const int64_t p = e % m, q = e / m;
+  double re = 0.0, im = 0.0;
+  for (int b = 0; b < k; ++b) {
+    const double2 tv = t[p + m * b];
+    const double2 xv = x[q + m * b];
+    re += tv.x * xv.x - tv.y * xv.y;
+    im += tv.x * xv.y + tv.y * xv.x;
+  }
+  out[e] = make_double2(re, im);
+}
+
+// W = Y X (or Y conj(X)) for the M x M matrix Y (a two-electron vector,
+// column q = electron-2 index contiguous) and the M x K orbital block X:
+// each thread owns one row p and all K outputs; X(q, .) is the same for the
+// whole workgroup (L1-resident), Y is read once, coalesced along p.
+template <int KM>
+__global__ __launch_bounds__(256) void k_ywx(const double2 *__restrict__ y, const double2 *__restrict__ x,
+                                             double2 *__restrict__ w, int64_t m, int k, int conj_x) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  double re[KM], im[KM];
+#pragma unroll
+  for (int b = 0; b < KM; ++b) re[b] = im[b] = 0.0;
+  const double sx = conj_x ? -1.0 : 1.0;
+  for (int64_t q = 0; q < m; ++q) {
+    const double2 yv = y[p + m * q];
+#pragma unroll
+    for (int b = 0; b < KM; ++b) {
+      if (b < k) {
+        const double2 xv = x[q + m * b];
+        const double xi = sx * xv.y;
+        re[b] += yv.x * xv.x - yv.y * xi;
+        im[b] += yv.x * xi + yv.y * xv.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < KM; ++b)
+    if (b < k) w[p + m * b] = make_double2(re[b], im[b]);
+}
+
 }  // namespace
+
+hipError_t launch_lowrank(const Dbl2 *t, const Dbl2 *x, Dbl2 *out, int64_t m, int k, hipStream_t s) {
+  if (k < 1 || m < 1) return hipErrorInvalidValue;
+  const int64_t n = m * m;
+  hipLaunchKernelGGL(k_lowrank, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const double2 *>(t), reinterpret_cast<const double2 *>(x),
+                     reinterpret_cast<double2 *>(out), m, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_ywx(const Dbl2 *y, const Dbl2 *x, Dbl2 *w, int64_t m, int k, int conj_x, hipStream_t s) {
+  if (k < 1 || k > 32 || m < 1) return hipErrorInvalidValue;
+  const dim3 grid(unsigned((m + 255) / 256)), block(256);
+  const double2 *yy = reinterpret_cast<const double2 *>(y), *xx = reinterpret_cast<const double2 *>(x);
+  double2 *ww = reinterpret_cast<double2 *>(w);
+  if (k <= 16)
+    hipLaunchKernelGGL(k_ywx<16>, grid, block, 0, s, yy, xx, ww, m, k, conj_x);
+  else
+    hipLaunchKernelGGL(k_ywx<32>, grid, block, 0, s, yy, xx, ww, m, k, conj_x);
+  return hipGetLastError();
+}
 
 int gram_blocks() { return kGramBlocks; }
 

@@ -199,6 +199,28 @@ class HeliumGrid:
         check(self._lib.dex_precblock_device(self._ctx, _dev(x), _dev(px), self.m, self.m, nev,
                                              float(sigma.real), float(sigma.imag), inner_iter, symmetry))
 
+    def diagsubspace_device(self, orbitals: np.ndarray, numout: int, etarget: complex, singlet: bool = True,
+                            hermopt: bool = False):
+        """DIAGSUBSPACE (hammod.F90:1125-1256) + GET_EXPECTS: start vectors from
+        the (anti)symmetrised products of one-electron orbitals (M x numorbs).
+        Returns (psi: torch complex128 of numout * D on the device, e0, expects)."""
+        import torch
+
+        orb = np.asarray(orbitals, dtype=np.complex128)
+        if orb.ndim != 2 or orb.shape[0] != self.m:
+            raise DexError(_lib.DEX_ESIZE, "orbitals must be an M x numorbs array")
+        k = orb.shape[1]
+        orb = np.ascontiguousarray(orb.reshape(-1, order="F"))  # column-major, flat
+        psi = torch.empty(self.d * numout, dtype=torch.complex128, device=f"cuda:{self.device}")
+        e0 = np.zeros(numout, dtype=np.complex128)
+        ex = np.zeros(numout, dtype=np.complex128)
+        check(self._lib.dex_diagsubspace_device(self._ctx, orb.view(np.float64).ctypes.data_as(_dp), k,
+                                                int(singlet), int(hermopt), float(np.real(etarget)),
+                                                float(np.imag(etarget)), numout, _dev(psi),
+                                                e0.view(np.float64).ctypes.data_as(_dp),
+                                                ex.view(np.float64).ctypes.data_as(_dp)))
+        return psi, e0, ex
+
     def footprint(self, nev: int, ns: int = 1, inner_iter: int = 0, inner_block: int = 0,
                   guard: int = 0) -> int:
         o = self._opts(False, False, 0, 2, 1e-5, 0, False, inner_block, guard)

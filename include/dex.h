@@ -189,6 +189,20 @@ int dex_precblock_device(dex_ctx *ctx, const void *x_dev, void *px_dev, int32_t 
                          int32_t nev, double sigma_re, double sigma_im, int32_t inner_iter,
                          int32_t symmetry);
 
+/* Physically targeted start vectors from one-electron orbitals:
+ * HELIUM_ORBITALGATHER's Gram-Schmidt of the orbitals (bilinear products, or
+ * Hermitian for hermopt != 0; hammod.F90:1080-1113), DIAGSUBSPACE over the
+ * (anti)symmetrised orbital products (singlet: a <= b, triplet: a < b;
+ * hammod.F90:1125-1256) with its eigenvalues ordered by |E0 - etarget| as
+ * mysort (hammod.F90:1658-1690), and GET_EXPECTS with the Hermitian product
+ * (cshiftopt false, hammod.F90:1261-1301).  orbitals: M x numorbs complex,
+ * column-major, host (the He+ eigenvectors the reference's Lanczos returns).
+ * psi_dev: numout device vectors (D complex each, Hermitian-normalised); e0,
+ * expects: numout complex each.  The grid needs the LDS matvec kernels. */
+int dex_diagsubspace_device(dex_ctx *ctx, const double *orbitals, int32_t numorbs, int32_t singlet,
+                            int32_t hermopt, double etarget_re, double etarget_im, int32_t numout,
+                            void *psi_dev, double *e0, double *expects);
+
 /* Bytes of device memory a dex_gplmr_eig_device solve of (nev, ns, inner_iter,
  * opts: inner_block and guard; NULL for the defaults) holds on this context,
  * the caller's block X included. */

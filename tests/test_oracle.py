@@ -196,3 +196,37 @@ def test_inner_solve_is_krylov_least_squares(order, k):
     r = hd @ x - sigma * x - b
     cn = np.linalg.norm(amat, axis=0)
     assert (np.abs(amat.conj().T @ r) / cn[:, None]).max() <= 1e-9 * np.linalg.norm(b)
+
+
+def test_orbital_subspace_matches_reference_log(golden_dir):
+    """DIAGSUBSPACE over the 14 lowest He+ orbitals (singlet products, bilinear
+    products, target -0.76) at NN=11, delta=2, theta=0.2 against the values the
+    reference printed (Out.S.N5:97-112).  This pins the whole two-electron
+    operator (kinetic, nuclear and e-e potential, scaling phases) against the
+    reference: an e-e term off by a factor, a wrong phase or a wrong Tinv gives
+    errors of 1e-2 and more.  The log's orbitals were Lanczos vectors converged
+    to LANTOL 1e-2 (the members of its degenerate groups differ by up to 1.3e-5),
+    so the bounds are 2e-5 per value, 1e-5 for the means over the log's
+    degenerate groups and 5e-6 for the non-degenerate first value."""
+    from oracle.c_oracle import CHammult
+    from oracle.startvec import diagsubspace, heplus_orbitals
+
+    g = json.loads((golden_dir / "heplus_nn11_d2_t0.2.json").read_text())
+    sub = g["subspace"]
+    h = Hamiltonian(5, 2.0, 0.2)
+    c = CHammult(h)
+    _, orb = heplus_orbitals(h, sub["numorbs"])
+    et = complex(*sub["etarget"])
+    e0, psi, ex = diagsubspace(lambda b: c(np.asfortranarray(b)), orb, True, False, et, 6)
+    logv = np.array([complex(a, b) for a, b in sub["eigenvalues"]])
+    assert np.max(np.abs(e0 - logv)) < 2e-5
+    assert abs(e0[0] - logv[0]) < 5e-6
+    assert abs(np.mean(e0[1:4]) - np.mean(logv[1:4])) < 1e-5
+    assert abs(np.mean(e0[4:6]) - np.mean(logv[4:6])) < 1e-5
+    assert abs(ex[0] - complex(*sub["first_expectation"])) < 1e-5
+    # the returned vectors are singlet and Hermitian-normalised
+    m = h.m
+    for j in range(psi.shape[1]):
+        v = psi[:, j].reshape(m, m, order="F")
+        assert np.abs(v - v.T).max() < 1e-12 * np.abs(v).max()
+        assert abs(np.vdot(psi[:, j], psi[:, j]) - 1.0) < 1e-12
