@@ -1,0 +1,46 @@
+"""Continuous-batching server on the GPU: each request's result is the same
+when it runs alone and when it runs in a mix with other requests in flight
+(own context and stream per request; the kernels' reductions have a fixed
+order, so the agreement is exact, and the test allows 1e-12)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dex():
+    import doublyexcited_amd as d
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return d
+
+
+def _requests():
+    from doublyexcited_amd.serve import Request
+
+    specs = [(2, 0.5, 0.30, 102), (3, 0.5, 0.25, 7), (2, 0.45, 0.35, 11), (3, 0.55, 0.30, 5)]
+    return [Request(arrival=0.01 * i, rid=i, n=n, delta=dl, theta=th, sigma=-0.78 - 0.002j, nev=4,
+                    tol=1e-8, maxit=400, seed=sd, inner_iter=4, guard=2)
+            for i, (n, dl, th, sd) in enumerate(specs)]
+
+
+def test_results_alone_equal_results_in_the_mix(dex):
+    from doublyexcited_amd.serve import Server
+
+    print("case: serve alone vs mix", flush=True)
+    reqs = _requests()
+    alone = [Server(slots=1).run([r], realtime=False)[0] for r in reqs]
+    mixed = Server(slots=4).run(reqs, realtime=False)
+    for a, b in zip(alone, mixed):
+        assert a.ok and b.ok, (a.error, b.error)
+        assert a.iterations == b.iterations
+        assert a.converged_pairs == b.converged_pairs
+        np.testing.assert_allclose(b.e, a.e, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(b.residuals, a.residuals, rtol=0, atol=1e-12)
+    # every converged pair meets the tolerance
+    for r in mixed:
+        assert np.sum(r.residuals <= 1.5e-8) == r.converged_pairs
