@@ -1,7 +1,13 @@
 #!/usr/bin/env python3
 """Benchmark: converged resonance eigenpairs/s and solve latency of the GPLMR /
-harmonic-Ritz solver on one MI355X, FP64 beside the mixed (FP32 matvec + FP64
-restarts) form.
+harmonic-Ritz solver on one MI355X, FP64 beside the mixed form.
+
+FP64 form: every matvec and block operation in FP64 (the inner solve too).
+Mixed form: the outer matvec in FP32 (complex128 vectors, complex64
+intermediate) until the residual reaches 1e-5, then FP64 restarts to the
+tolerance (with --mixed-inner-fp32 also the inner solve's Arnoldi basis in
+complex64 with FP32 matvecs).  Orthogonalisation, the projected eigenproblem
+and the final residuals stay FP64 in both forms.
 
 A step is one complete resonance solve of the selected workload: a seeded
 complex-Gaussian singlet start block taken by GPLMR (with the in-house inner
@@ -74,7 +80,7 @@ class Slot:
         self.lock = threading.Lock()
 
 
-def solve(torch, slot: Slot, w: dict, x, timing: bool) -> dict:
+def solve(torch, slot: Slot, w: dict, x, timing: bool, inner_fp32: bool = False) -> dict:
     import numpy as np
 
     g = slot.g
@@ -85,8 +91,7 @@ def solve(torch, slot: Slot, w: dict, x, timing: bool) -> dict:
         t0 = time.time()
         r = g.gplmr_eig_device(x, w["sigma"], w["nev"], ns=1, maxit=w["maxit"], tol=w["tol"],
                                inner_iter=w["inner"], inner_block=w["inner_block"], guard=w["guard"],
-                               symmetry=1,
-                               fp32_tol=max(w["tol"], 1e-5))
+                               symmetry=1, fp32_tol=max(w["tol"], 1e-5), inner_fp32=inner_fp32)
         lat = time.time() - t0
         stats = g.kernel_stats() if timing else {}
         g.set_kernel_timing(False)
@@ -97,7 +102,8 @@ def solve(torch, slot: Slot, w: dict, x, timing: bool) -> dict:
                 restarts=r.fp64_restarts_used, stats=stats)
 
 
-def run_form(dex, torch, w: dict, precision: str, seeds, concurrency: int, timing: bool, slots=None):
+def run_form(dex, torch, w: dict, precision: str, seeds, concurrency: int, timing: bool, slots=None,
+             inner_fp32: bool = False):
     """Solves one start block per seed, up to `concurrency` at once; returns
     (results in seed order, wall seconds of the solves, slots)."""
     if slots is None:
@@ -112,7 +118,7 @@ def run_form(dex, torch, w: dict, precision: str, seeds, concurrency: int, timin
     out = [None] * len(seeds)
 
     def job(i):
-        out[i] = solve(torch, slots[i % len(slots)], w, blocks[i], timing)
+        out[i] = solve(torch, slots[i % len(slots)], w, blocks[i], timing, inner_fp32=inner_fp32)
         r = out[i]
         log(f"  {precision} solve seed {seeds[i]}: {r['lat']:.2f}s iters {r['iters']} matvecs {r['matvecs']} "
             f"converged {r['conv']}/{w['nev']} max res {r['res'].max():.2e}")
@@ -197,6 +203,8 @@ def main() -> int:
                     choices=sorted(WORKLOADS))
     ap.add_argument("--concurrency", type=int, default=0, help="solves in flight (0: workload default)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-inner-fp32", action="store_true",
+                    help="mixed form: complex64 Arnoldi basis in the inner solve as well")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -247,7 +255,8 @@ def main() -> int:
     }
     if not args.no_mixed:
         log(f"mixed form: {args.steps} solves, {conc} in flight")
-        resmx, wall_mx, mslots = run_form(dex, torch, w, "mixed", seeds, conc, False)
+        resmx, wall_mx, mslots = run_form(dex, torch, w, "mixed", seeds, conc, False,
+                                          inner_fp32=args.mixed_inner_fp32)
         for s in mslots:
             s.g.close()
         de = []

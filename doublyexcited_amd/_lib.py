@@ -65,7 +65,7 @@ class DexSolveOpts(C.Structure):
     _fields_ = [("strict_cholesky", C.c_int32), ("verbose", C.c_int32),
                 ("refresh_every", C.c_int32), ("fp64_restarts", C.c_int32),
                 ("fp32_tol", C.c_double), ("symmetry", C.c_int32), ("lock", C.c_int32),
-                ("inner_block", C.c_int32), ("guard", C.c_int32)]
+                ("inner_block", C.c_int32), ("guard", C.c_int32), ("inner_fp32", C.c_int32)]
 
 
 class DexError(RuntimeError):

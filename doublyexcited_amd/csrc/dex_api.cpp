@@ -182,15 +182,35 @@ int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0) {
   return 0;
 }
 
+// H on complex64 device vectors in FP32 arithmetic (x, y hold D complex64).
+int apply_h32(dex_ctx *c, const Dbl2 *x, Dbl2 *y, int sym) {
+  if (int st = ensure_u(c)) return st;
+  if (!c->fast32) return fail(DEX_EUNSUPPORTED, "no FP32 matvec for this grid");
+  const double dd = double(c->d), nn = double(c->nn);
+  {
+    KTimer kt(c, DEX_K_PASS1, 16.0 * dd, 12.0 * nn * dd);
+    DEX_HIP(dex::launch_pass1(c->nn, dex::kIoF32, x, c->d_u, c->stream));
+  }
+  {
+    const double fl = sym != 0 ? 24.0 * dd : (12.0 * nn + 14.0) * dd;
+    KTimer kt(c, sym != 0 ? DEX_K_SYMADD : DEX_K_PASS2, 24.0 * dd, fl);
+    DEX_HIP(second_pass(c, dex::kIoF32, x, y, sym));
+  }
+  return 0;
+}
+
 // ---- block helpers on one context ------------------------------------------
+// Columns of a block (device vectors of D entries) and their storage:
+// fmt 0 complex128, 1 complex64 (a complex64 column uses half of its buffer).
 struct Block {
   std::vector<Dbl2 *> cols;
+  int fmt = 0;
   int size() const { return int(cols.size()); }
 };
 
-ColDesc plain(const Dbl2 *a) { return ColDesc{a, nullptr, 0.0, 0.0, -1, 0}; }
-ColDesc shifted(const Dbl2 *a, const Dbl2 *b, cd sig) {
-  return ColDesc{a, b, sig.real(), sig.imag(), -1, 0};
+ColDesc plain(const Dbl2 *a, int fmt = 0) { return ColDesc{a, nullptr, 0.0, 0.0, -1, fmt}; }
+ColDesc shifted(const Dbl2 *a, const Dbl2 *b, cd sig, int fmt = 0) {
+  return ColDesc{a, b, sig.real(), sig.imag(), -1, fmt};
 }
 
 int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
@@ -279,12 +299,12 @@ int gram_harmonic(dex_ctx *c, const std::vector<Dbl2 *> &q, const std::vector<Db
 
 // out_j = sum_i in_i C(i, j) (+ beta_j out_j), row-local, in-place safe.
 int combine(dex_ctx *c, const std::vector<ColDesc> &in, const std::vector<Dbl2 *> &out,
-            const std::vector<cd> &cm, const std::vector<cd> *beta = nullptr) {
+            const std::vector<cd> &cm, const std::vector<cd> *beta = nullptr, int out_fmt = 0) {
   const int nin = int(in.size()), nout = int(out.size());
   if (nout == 0) return 0;
   if (nin * nout > kMaxSmall) return fail(DEX_EINVAL, "combine matrix too large");
   std::vector<ColDesc> ds(in);
-  for (Dbl2 *o : out) ds.push_back(plain(o));
+  for (Dbl2 *o : out) ds.push_back(plain(o, out_fmt));
   if (int st = upload_descs(c, ds)) return st;
   DEX_HIP(hipMemcpyAsync(c->d_small, cm.data(), cm.size() * sizeof(Dbl2), hipMemcpyHostToDevice,
                          c->stream));
@@ -333,7 +353,7 @@ int coldots(dex_ctx *c, const std::vector<ColDesc> &a, const std::vector<ColDesc
 
 std::vector<ColDesc> descs_of(const Block &b) {
   std::vector<ColDesc> v;
-  for (Dbl2 *p : b.cols) v.push_back(plain(p));
+  for (Dbl2 *p : b.cols) v.push_back(plain(p, b.fmt));
   return v;
 }
 
@@ -664,12 +684,17 @@ struct Solver {
   int fallbacks = 0;
   int64_t matvecs = 0;
   double t_matvec = 0, t_gram = 0, t_update = 0, t_small = 0, t_prec = 0;
+  bool inner32 = false;  // inner solve: complex64 Arnoldi basis and FP32 matvecs
   std::vector<Dbl2 *> pool;
 
   int matblock(const Block &in, const Block &out) {
     const double t0 = now_s();
     for (int j = 0; j < in.size(); ++j) {
-      if (int st = apply_h(c, in.cols[j], out.cols[j], single, sym)) return st;
+      if (in.fmt != 0) {
+        if (int st = apply_h32(c, in.cols[j], out.cols[j], sym)) return st;
+      } else if (int st = apply_h(c, in.cols[j], out.cols[j], single, sym)) {
+        return st;
+      }
       ++matvecs;
     }
     DEX_HIP(hipStreamSynchronize(c->stream));
@@ -758,11 +783,11 @@ struct Solver {
       if (kout > 0) {
         std::vector<ColDesc> in = descs_of(x);
         std::vector<Dbl2 *> out(x.cols.begin(), x.cols.begin() + kout);
-        if (int st = combine(c, in, out, t)) return -st;
+        if (int st = combine(c, in, out, t, nullptr, x.fmt)) return -st;
         if (ax) {
           std::vector<ColDesc> ain = descs_of(*ax);
           std::vector<Dbl2 *> aout(ax->cols.begin(), ax->cols.begin() + kout);
-          if (int st = combine(c, ain, aout, t)) return -st;
+          if (int st = combine(c, ain, aout, t, nullptr, ax->fmt)) return -st;
         }
       }
       t_update += now_s() - t0;
@@ -801,13 +826,13 @@ struct Solver {
       }
       std::vector<ColDesc> in;
       if (first) in = descs_of(b);
-      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i]));
-      if (int st = combine(c, in, b.cols, cm, first ? nullptr : &one)) return st;
+      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i], q.fmt));
+      if (int st = combine(c, in, b.cols, cm, first ? nullptr : &one, b.fmt)) return st;
       if (ab) {
         std::vector<ColDesc> ain;
         if (first) ain = descs_of(*ab);
-        for (int i = 0; i < nq; ++i) ain.push_back(plain(aq->cols[q0 + i]));
-        if (int st = combine(c, ain, ab->cols, cm, first ? nullptr : &one)) return st;
+        for (int i = 0; i < nq; ++i) ain.push_back(plain(aq->cols[q0 + i], aq->fmt));
+        if (int st = combine(c, ain, ab->cols, cm, first ? nullptr : &one, ab->fmt)) return st;
       }
       q0 += nq;
     }
@@ -824,12 +849,12 @@ struct Solver {
     const double t0 = now_s();
     const int n1 = std::min(kq, 64 - k);
     std::vector<ColDesc> ds(w);
-    for (int i = 0; i < n1; ++i) ds.push_back(plain(q.cols[i]));
+    for (int i = 0; i < n1; ++i) ds.push_back(plain(q.cols[i], q.fmt));
     if (int st = gram_launch_copy(c, ds, k + n1, k, k, g, rows, 0, 0)) return st;
     for (int i0 = n1; i0 < kq; i0 += 64) {
       const int bl = std::min(64, kq - i0);
       std::vector<ColDesc> d2;
-      for (int i = 0; i < bl; ++i) d2.push_back(plain(q.cols[i0 + i]));
+      for (int i = 0; i < bl; ++i) d2.push_back(plain(q.cols[i0 + i], q.fmt));
       d2.insert(d2.end(), w.begin(), w.end());
       if (int st = gram_launch_copy(c, d2, bl, k, 0, g, rows, k + i0, 0)) return st;
     }
@@ -867,8 +892,8 @@ struct Solver {
       }
       std::vector<ColDesc> in;
       if (first) in = w;
-      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i]));
-      if (int st = combine(c, in, out.cols, cm, first ? nullptr : &one)) return st;
+      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i], q.fmt));
+      if (int st = combine(c, in, out.cols, cm, first ? nullptr : &one, out.fmt)) return st;
       q0 += nq;
       if (nq == 0) break;
     }
@@ -1036,9 +1061,12 @@ struct Solver {
     const int k = b.size();
     if (k == 0 || m < 1) return 0;
     const double t0 = now_s();
+    const int bf = inner32 ? 1 : 0;  // storage of the Arnoldi basis
     Block qall;
+    qall.fmt = bf;
     // Q_1 R0 = b (into fresh storage: b is overwritten only by the solution)
     Block q1;
+    q1.fmt = bf;
     if (int st = grow(q1, k)) return st;
     std::vector<cd> cq, r0;
     bool ok = false;
@@ -1066,12 +1094,14 @@ struct Solver {
     int nblk = 0;
     for (int j = 0; j < m; ++j) {
       Block qj;
+      qj.fmt = bf;
       qj.cols.assign(qall.cols.end() - kb, qall.cols.end());
       Block hq;
+      hq.fmt = bf;
       if (int st = grow(hq, kb)) return st;
       if (int st = matblock(qj, hq)) return st;
       std::vector<ColDesc> wd;
-      for (int i = 0; i < kb; ++i) wd.push_back(shifted(hq.cols[i], qj.cols[i], sigma));
+      for (int i = 0; i < kb; ++i) wd.push_back(shifted(hq.cols[i], qj.cols[i], sigma, bf));
       std::vector<cd> cj, rj;
       const bool last = j == m - 1;
       // a preconditioner's basis: one pass unless the projection cancels
@@ -1110,6 +1140,7 @@ struct Solver {
     for (int cc = 0; cc < k; ++cc)
       for (int i = 0; i < cols; ++i) y[i + size_t(cols) * cc] = rhs[i + size_t(rows) * cc];
     Block qm;
+    qm.fmt = bf;
     qm.cols.assign(qall.cols.begin(), qall.cols.begin() + cols);
     const double tu = now_s();
     if (int st = combine_chunked(qm, y, b)) return st;
@@ -1126,12 +1157,12 @@ struct Solver {
     for (int i0 = 0; i0 < nin; i0 += 64) {
       const int bi = std::min(64, nin - i0);
       std::vector<ColDesc> ds;
-      for (int i = 0; i < bi; ++i) ds.push_back(plain(in.cols[i0 + i]));
+      for (int i = 0; i < bi; ++i) ds.push_back(plain(in.cols[i0 + i], in.fmt));
       std::vector<cd> sub(size_t(bi) * nout);
       for (int j = 0; j < nout; ++j)
         for (int i = 0; i < bi; ++i) sub[i + size_t(bi) * j] = cm[i0 + i + size_t(nin) * j];
       std::vector<cd> one(nout, 1.0);
-      if (int st = combine(c, ds, out.cols, sub, i0 > 0 ? &one : nullptr)) return st;
+      if (int st = combine(c, ds, out.cols, sub, i0 > 0 ? &one : nullptr, out.fmt)) return st;
     }
     return 0;
   }
@@ -1170,6 +1201,7 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
               bool single, std::vector<Dbl2 *> &store) {
   const int nev = int(xcols.size());
   Solver S{c, nev, ns, o.strict_cholesky != 0, single, o.symmetry};
+  S.inner32 = o.inner_fp32 != 0;
   // every buffer in `store` is free at entry
   const size_t vbytes = size_t(c->d) * sizeof(Dbl2);
   // columns: X in xcols; the rest from `store` (pool of (5 + 2 ns) nev vectors)
@@ -1452,6 +1484,8 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
                                     const dex_solve_opts *opts, dex_result *res) {
   if (!c || !x_dev || !e) return fail(DEX_EINVAL, "null argument");
   if (int64_t(m) * mb != c->d) return fail(DEX_ESIZE, "SIZE ERROR: M*MB != NN^6");
+  if (opts && opts->inner_fp32 != 0 && !c->fast32)
+    return fail(DEX_EUNSUPPORTED, "inner_fp32 needs the FP32 matvec kernels for this grid");
   const int guard = opts ? opts->guard : 0;
   if (guard < 0) return fail(DEX_EINVAL, "guard must be >= 0");
   if (nev < 1 || ns < 1 || (3 + ns) * (nev + guard) > 64)

@@ -45,7 +45,7 @@ struct ColDesc {
   const Dbl2 *b;
   double sig_re, sig_im;
   int32_t bslot;
-  int32_t pad;
+  int32_t fmt;  // storage of a and b (and of an output column): 0 complex128, 1 complex64
 };
 
 struct MatvecCoef {

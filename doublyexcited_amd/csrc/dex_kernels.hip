@@ -556,127 +556,46 @@ hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *v
 namespace {
 
 constexpr int kGramThreads = 256;
-#ifndef DEX_GRAM_ROWS
-#define DEX_GRAM_ROWS 16
-#endif
-constexpr int kGramRowsMax = DEX_GRAM_ROWS;  // rows per LDS tile (at most)
-
-// Rows per LDS tile for an (MA, NB) Gram: the largest power of two <=
-// kGramRowsMax whose tile fits 48 KB (three workgroups per CU), at least 16.
-constexpr int gram_rows(int ma, int nb) {
-  int r = kGramRowsMax;
-  while (r > 16 && r * (16 * ma + 16 * nb + 1) * 16 > 48 * 1024) r /= 2;
-  return r;
-}
 constexpr int kGramMaxL = 64;
 constexpr int kGramMaxR = 128;
 constexpr int kGramBlocks = 1024;
 
+// Element r of a complex128 (fmt 0) or complex64 (fmt 1) column, in FP64.
+__device__ __forceinline__ double2 ld_col(const void *p, int fmt, int64_t r) {
+  if (fmt != 0) {
+    const float2 v = reinterpret_cast<const float2 *>(p)[r];
+    return make_double2(double(v.x), double(v.y));
+  }
+  return reinterpret_cast<const double2 *>(p)[r];
+}
+
+__device__ __forceinline__ void st_col(void *p, int fmt, int64_t r, double2 v) {
+  if (fmt != 0)
+    reinterpret_cast<float2 *>(p)[r] = make_float2(float(v.x), float(v.y));
+  else
+    reinterpret_cast<double2 *>(p)[r] = v;
+}
+
 __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
-  const double2 va = reinterpret_cast<const double2 *>(c.a)[r];
+  const double2 va = ld_col(c.a, c.fmt, r);
   if (c.b == nullptr) return va;
-  const double2 vb = reinterpret_cast<const double2 *>(c.b)[r];
+  const double2 vb = ld_col(c.b, c.fmt, r);
   double2 o;
   o.x = va.x - (c.sig_re * vb.x - c.sig_im * vb.y);
   o.y = va.y - (c.sig_re * vb.y + c.sig_im * vb.x);
   return o;
 }
 
-// Partial Gram G = L^H R over a contiguous row range per workgroup.  The 256
-// threads form a 16 x 16 grid; thread (a, b) owns the outputs
-// (a + 16 m, b + 16 n), m < MA, n < NB, so each LDS row read is shared by many
-// FMAs and consecutive lanes read consecutive columns.  LDS holds kGramRows
-// rows: L in slots [0, 16 MA), the R columns not shared with L after it.
-// R column j < share is L column j.  For share > 0 the blocks (m, n) with
-// m > n that lie wholly inside the leading share x share part (Hermitian:
-// L^H L there) are skipped.
-template <int MA, int NB>
-__global__ __launch_bounds__(kGramThreads) void k_gram(const ColDesc *__restrict__ descs, int kl,
-                                                       int kr, int share, int64_t d,
-                                                       int64_t rows_per_block,
-                                                       double2 *__restrict__ partial) {
-  constexpr int LD = 16 * MA + 16 * NB + 1;
-  constexpr int kGramRows = gram_rows(MA, NB);
-  __shared__ double2 tile[kGramRows * LD];
-  const int tid = threadIdx.x;
-  const int a = tid % 16, b = tid / 16;
-  const int nload = kl + (kr - share);
-  int rslot[NB];
-#pragma unroll
-  for (int n = 0; n < NB; ++n) {
-    const int j = b + 16 * n;
-    rslot[n] = j < share ? j : 16 * MA + (j - share);
-  }
-  double accr[MA][NB], acci[MA][NB];
-#pragma unroll
-  for (int m = 0; m < MA; ++m)
-#pragma unroll
-    for (int n = 0; n < NB; ++n) accr[m][n] = acci[m][n] = 0.0;
-  const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
-  const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
-  for (int64_t rb = r0; rb < r1; rb += kGramRows) {
-    const int nr = int(r1 - rb < kGramRows ? r1 - rb : kGramRows);
-    for (int e = tid; e < kGramRows * nload; e += kGramThreads) {
-      const int r = e % kGramRows, c = e / kGramRows;
-      const ColDesc dc = descs[c];
-      if (dc.a == nullptr) continue;  // supplied through another column's bslot
-      const int slot = c < kl ? c : 16 * MA + (c - kl);
-      if (r < nr) {
-        const int64_t row = rb + r;
-        const double2 va = reinterpret_cast<const double2 *>(dc.a)[row];
-        if (dc.b == nullptr) {
-          tile[r * LD + slot] = va;
-        } else {
-          const double2 vb = reinterpret_cast<const double2 *>(dc.b)[row];
-          double2 o;
-          o.x = va.x - (dc.sig_re * vb.x - dc.sig_im * vb.y);
-          o.y = va.y - (dc.sig_re * vb.y + dc.sig_im * vb.x);
-          tile[r * LD + slot] = o;
-          if (dc.bslot >= 0) tile[r * LD + dc.bslot] = vb;
-        }
-      } else {
-        tile[r * LD + slot] = make_double2(0.0, 0.0);
-        if (dc.b != nullptr && dc.bslot >= 0) tile[r * LD + dc.bslot] = make_double2(0.0, 0.0);
-      }
-    }
-    __syncthreads();
-#pragma unroll 2
-    for (int r = 0; r < kGramRows; ++r) {
-      double2 lv[MA], rv[NB];
-#pragma unroll
-      for (int m = 0; m < MA; ++m) lv[m] = tile[r * LD + a + 16 * m];
-#pragma unroll
-      for (int n = 0; n < NB; ++n) rv[n] = tile[r * LD + rslot[n]];
-#pragma unroll
-      for (int m = 0; m < MA; ++m)
-#pragma unroll
-        for (int n = 0; n < NB; ++n) {
-          if (m > n && 16 * (m + 1) <= share) continue;
-          accr[m][n] += lv[m].x * rv[n].x + lv[m].y * rv[n].y;
-          acci[m][n] += lv[m].x * rv[n].y - lv[m].y * rv[n].x;
-        }
-    }
-    __syncthreads();
-  }
-  double2 *out = partial + size_t(blockIdx.x) * kl * kr;
-#pragma unroll
-  for (int m = 0; m < MA; ++m)
-#pragma unroll
-    for (int n = 0; n < NB; ++n) {
-      const int i = a + 16 * m, j = b + 16 * n;
-      if (i < kl && j < kr) out[i + size_t(kl) * j] = make_double2(accr[m][n], acci[m][n]);
-    }
-}
-
-// The same partial Gram on the FP64 matrix cores.  G = L^H R is a GEMM whose
+// Partial Gram G = L^H R over a contiguous row range per workgroup, on the FP64
+// matrix cores.  G = L^H R is a GEMM whose
 // inner dimension is the vector length: per 4 rows, v_mfma_f64_16x16x4f64
 // multiplies a 16 x 4 slice of L^T by a 4 x 16 slice of R into a 16 x 16
 // output tile (lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and
 // holds D[4 v + l / 16][l % 16], v < 4).  A complex tile takes four real
 // products: Re += Lr Rr + Li Ri, Im += Lr Ri - Li Rr.  The rows are staged
-// through LDS as in k_gram (one coalesced pass over every column); each LDS
-// value then feeds 16 MACs of a matrix instruction instead of one vector FMA,
-// which is what limited k_gram to the LDS bandwidth.  The 4 waves of a
+// through LDS (one coalesced pass over every column); each LDS value then
+// feeds 16 MACs of a matrix instruction instead of one vector FMA (a vector-FMA
+// version of this kernel was bound by the LDS bandwidth at ~2.5 TB/s).  The 4 waves of a
 // workgroup share the output tiles (MT x NT of them) round-robin.
 typedef double dex_v4d __attribute__((ext_vector_type(4)));
 
@@ -721,17 +640,20 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
   }
   // Element p of this thread is (row e % ROWS, column e / ROWS) of every
   // tile, e = tid + 256 p: its column pointers are fixed for the whole loop.
-  const double2 *pa[PMAX];
-  const double2 *pb[PMAX];
+  const void *pa[PMAX];
+  const void *pb[PMAX];
+  int pf[PMAX];
 #pragma unroll
   for (int p = 0; p < PMAX; ++p) {
     const int e = tid + kGramThreads * p;
     pa[p] = nullptr;
     pb[p] = nullptr;
+    pf[p] = 0;
     if (e < ROWS * nload) {
       const ColDesc &dc = descs[e / ROWS];
-      pa[p] = reinterpret_cast<const double2 *>(dc.a);
-      pb[p] = reinterpret_cast<const double2 *>(dc.b);
+      pa[p] = dc.a;
+      pb[p] = dc.b;
+      pf[p] = dc.fmt;
     }
   }
   const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
@@ -744,8 +666,8 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
       va[p] = make_double2(0.0, 0.0);
       vb[p] = make_double2(0.0, 0.0);
       if (pa[p] != nullptr && row < r1) {
-        va[p] = pa[p][row];
-        if (pb[p] != nullptr) vb[p] = pb[p][row];
+        va[p] = ld_col(pa[p], pf[p], row);
+        if (pb[p] != nullptr) vb[p] = ld_col(pb[p], pf[p], row);
       }
     }
   };
@@ -853,17 +775,20 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
     sc[e] = j < nout ? cm[i + size_t(nin) * j] : make_double2(0.0, 0.0);
   }
   // element p of this thread: row e % kCombRows of input column e / kCombRows
-  const double2 *pa[PMAX];
-  const double2 *pb[PMAX];
+  const void *pa[PMAX];
+  const void *pb[PMAX];
+  int pf[PMAX];
 #pragma unroll
   for (int p = 0; p < PMAX; ++p) {
     const int e = tid + kCombThreads * p;
     pa[p] = nullptr;
     pb[p] = nullptr;
+    pf[p] = 0;
     if (e < kCombRows * nin) {
       const ColDesc &dc = descs[e / kCombRows];
-      pa[p] = reinterpret_cast<const double2 *>(dc.a);
-      pb[p] = reinterpret_cast<const double2 *>(dc.b);
+      pa[p] = dc.a;
+      pb[p] = dc.b;
+      pf[p] = dc.fmt;
     }
   }
   double2 va[PMAX], vb[PMAX];
@@ -872,8 +797,8 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
     for (int p = 0; p < PMAX; ++p) {
       const int64_t r = rb + (tid + kCombThreads * p) % kCombRows;
       if (pa[p] != nullptr && r < d) {
-        va[p] = pa[p][r];
-        if (pb[p] != nullptr) vb[p] = pb[p][r];
+        va[p] = ld_col(pa[p], pf[p], r);
+        if (pb[p] != nullptr) vb[p] = ld_col(pb[p], pf[p], r);
       }
     }
   };
@@ -920,15 +845,17 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
       for (int q = 0; q < JPT; ++q) {
         const int j = grp * JPT + q;
         if (j < nout) {
-          double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + j].a)) + rb + row;
+          const ColDesc &od = descs[nin + j];
+          void *o = const_cast<Dbl2 *>(od.a);
+          const int64_t r = rb + row;
           double ox = ax[q], oy = ay[q];
           if (beta != nullptr) {
             const double2 bj = beta[j];
-            const double2 ov = *o;
+            const double2 ov = ld_col(o, od.fmt, r);
             ox += bj.x * ov.x - bj.y * ov.y;
             oy += bj.x * ov.y + bj.y * ov.x;
           }
-          *o = make_double2(ox, oy);
+          st_col(o, od.fmt, r, make_double2(ox, oy));
         }
       }
     }
@@ -959,9 +886,9 @@ __global__ __launch_bounds__(256) void k_combine_w(const ColDesc *__restrict__ d
       const ColDesc dc = descs[i];
       double2 v = make_double2(0.0, 0.0);
       if (in_range) {
-        v = reinterpret_cast<const double2 *>(dc.a)[r];
+        v = ld_col(dc.a, dc.fmt, r);
         if (dc.b != nullptr) {
-          const double2 vb = reinterpret_cast<const double2 *>(dc.b)[r];
+          const double2 vb = ld_col(dc.b, dc.fmt, r);
           v.x -= dc.sig_re * vb.x - dc.sig_im * vb.y;
           v.y -= dc.sig_re * vb.y + dc.sig_im * vb.x;
         }
@@ -981,15 +908,16 @@ __global__ __launch_bounds__(256) void k_combine_w(const ColDesc *__restrict__ d
 #pragma unroll
     for (int q = 0; q < JN; ++q) {
       if (q < nout) {
-        double2 *o = reinterpret_cast<double2 *>(const_cast<Dbl2 *>(descs[nin + q].a)) + r;
+        const ColDesc &od = descs[nin + q];
+        void *o = const_cast<Dbl2 *>(od.a);
         double ox = ax[q], oy = ay[q];
         if (beta != nullptr) {
           const double2 bq = beta[q];
-          const double2 ov = *o;
+          const double2 ov = ld_col(o, od.fmt, r);
           ox += bq.x * ov.x - bq.y * ov.y;
           oy += bq.x * ov.y + bq.y * ov.x;
         }
-        *o = make_double2(ox, oy);
+        st_col(o, od.fmt, r, make_double2(ox, oy));
       }
     }
   }

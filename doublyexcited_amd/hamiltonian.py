@@ -229,13 +229,15 @@ class HeliumGrid:
     # -- eigensolver --------------------------------------------------------------------
     @staticmethod
     def _opts(strict: bool, verbose: bool, refresh_every: int, fp64_restarts: int, fp32_tol: float,
-              symmetry: int = 0, lock: bool = False, inner_block: int = 0, guard: int = 0):
+              symmetry: int = 0, lock: bool = False, inner_block: int = 0, guard: int = 0,
+              inner_fp32: bool = False):
         o = _lib.DexSolveOpts()
         _lib.load().dex_solve_opts_default(C.byref(o))
         o.symmetry = int(symmetry)
         o.lock = int(lock)
         o.inner_block = int(inner_block)
         o.guard = int(guard)
+        o.inner_fp32 = int(inner_fp32)
         o.strict_cholesky = int(strict)
         o.verbose = int(verbose)
         o.refresh_every = refresh_every
@@ -255,7 +257,7 @@ class HeliumGrid:
                   verbose: bool = False, refresh_every: int = 0, fp64_restarts: int = 2,
                   fp32_tol: float = 1e-5, m: int | None = None, mb: int | None = None,
                   symmetry: int = 0, lock: bool = False, inner_block: int = 0,
-                  guard: int = 0) -> SolveResult:
+                  guard: int = 0, inner_fp32: bool = False) -> SolveResult:
         """GPLMR_EIG on a host block ``x`` (D*NEV complex128, vector-major), updated in place."""
         if x.dtype != np.complex128 or not (x.flags.f_contiguous or x.flags.c_contiguous):
             raise TypeError("x must be a contiguous complex128 array")
@@ -266,7 +268,7 @@ class HeliumGrid:
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
         o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock, inner_block,
-                       guard)
+                       guard, inner_fp32)
         st = self._lib.dex_gplmr_eig(self._ctx, _ptr(x), e.view(np.float64).ctypes.data_as(_dp), m, mb,
                                      float(np.real(sigma)), float(np.imag(sigma)), nev, ns, maxit, tol,
                                      inner_iter, outmodulus, C.byref(o), C.byref(r))
@@ -278,12 +280,12 @@ class HeliumGrid:
                          strict: bool = False, verbose: bool = False, refresh_every: int = 0,
                          fp64_restarts: int = 2, fp32_tol: float = 1e-5,
                          symmetry: int = 0, lock: bool = False, inner_block: int = 0,
-                         guard: int = 0) -> SolveResult:
+                         guard: int = 0, inner_fp32: bool = False) -> SolveResult:
         """GPLMR_EIG with the block resident on the device (torch complex128 tensor)."""
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
         o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock, inner_block,
-                       guard)
+                       guard, inner_fp32)
         st = self._lib.dex_gplmr_eig_device(self._ctx, _dev(x),
                                             e.view(np.float64).ctypes.data_as(_dp), self.m, self.m,
                                             float(np.real(sigma)), float(np.imag(sigma)), nev, ns,

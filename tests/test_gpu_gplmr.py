@@ -164,6 +164,27 @@ def test_guard_columns_return_nearest_pairs(dex):
         assert np.linalg.norm(h.hammult(xj, v) - rg.e[j] * xj) <= 10 * 1e-10 * np.linalg.norm(xj)
 
 
+def test_fp32_inner_solve_same_pairs(dex):
+    """The inner solve with a complex64 Arnoldi basis and FP32 matvecs is only a
+    different preconditioner: the outer FP64 iteration returns the same pairs
+    (|dE| <= 1e-9, residuals recomputed in FP64 <= tol)."""
+    print("case: inner fp32 N=2", flush=True)
+    h = Hamiltonian(2, 0.5, 0.3)
+    v = h.potential2()
+    nev, sigma, tol = 4, -0.78 - 0.002j, 1e-10
+    x0 = start_block(h, nev, 102)
+    with dex.HeliumGrid(2, 0.5, 0.3) as g:
+        x = x0.copy(order="F")
+        r64 = g.gplmr_eig(x, sigma, nev, ns=1, maxit=400, tol=tol, inner_iter=4, symmetry=1)
+        x32 = x0.copy(order="F")
+        r32 = g.gplmr_eig(x32, sigma, nev, ns=1, maxit=400, tol=tol, inner_iter=4, symmetry=1, inner_fp32=True)
+    assert r64.converged and r32.converged
+    assert match(r32.e, r64.e) < 1e-9 and match(r64.e, r32.e) < 1e-9
+    for j in range(nev):
+        xj = x32[:, j]
+        assert np.linalg.norm(h.hammult(xj, v) - r32.e[j] * xj) <= 10 * tol * np.linalg.norm(xj)
+
+
 def test_mixed_precision_within_1e7(dex):
     """FP32 matvec phase to 1e-5, then FP64 restarts to 1e-9: |dE| <= 1e-7 au
     against the FP64 solve from the same start block."""
