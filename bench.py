@@ -51,18 +51,18 @@ RIDGE = FP64_PEAK_TFS * 1e12 / (HBM_PEAK_GBS * 1e9)  # flop/B where FP64 compute
 WORKLOADS = {
     # the plain CPU-runnable configuration of the reference method
     "he-nn11-block8": dict(n=5, delta=0.5, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-8,
-                           seed=1001, inner=8, inner_block=0, guard=2, maxit=800, concurrency=1,
+                           seed=1001, ns=3, inner=8, inner_block=0, guard=2, maxit=800, concurrency=1,
                            desc="He, NN=11 (D=1.77M), delta 0.5, theta 0.3, 8 eigenpairs nearest "
                                 "sigma -0.78-0.002i (block 8 + 2 guard columns), tol 1e-8, "
-                                "inner solve order 8"),
+                                "NS 3, inner solve order 8"),
     # the drop-in FP64 GPU configuration (a solve takes many minutes)
     "he-nn21-block16": dict(n=10, delta=0.4, theta=0.35, nev=16, sigma=complex(-0.70, -0.001), tol=1e-9,
-                            seed=1002, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
+                            seed=1002, ns=1, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
                             desc="He, NN=21 (D=85.8M), delta 0.4, theta 0.35, block 16, "
                                  "sigma -0.70-0.001i, tol 1e-9, inner solve order 4 on 4-column groups"),
     # the mixed-precision configuration (FP64 basis: 2 restarts after the FP32 phase)
     "he-nn25-block8": dict(n=12, delta=0.35, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-7,
-                           seed=1003, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
+                           seed=1003, ns=1, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
                            desc="He, NN=25 (D=244M), delta 0.35, theta 0.3, block 8, "
                                 "sigma -0.78-0.002i, tol 1e-7, inner solve order 4 on 4-column groups"),
 }
@@ -89,7 +89,7 @@ def solve(torch, slot: Slot, w: dict, x, timing: bool, inner_fp32: bool = False)
         if timing:
             g.reset_kernel_stats()
         t0 = time.time()
-        r = g.gplmr_eig_device(x, w["sigma"], w["nev"], ns=1, maxit=w["maxit"], tol=w["tol"],
+        r = g.gplmr_eig_device(x, w["sigma"], w["nev"], ns=w["ns"], maxit=w["maxit"], tol=w["tol"],
                                inner_iter=w["inner"], inner_block=w["inner_block"], guard=w["guard"],
                                symmetry=1, fp32_tol=max(w["tol"], 1e-5), inner_fp32=inner_fp32)
         lat = time.time() - t0
