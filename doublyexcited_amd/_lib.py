@@ -38,6 +38,7 @@ EXPORTS = [
     "dex_status_string", "dex_last_error", "dex_set_kernel_timing", "dex_get_kernel_timing",
     "dex_get_kernel_stats", "dex_reset_kernel_stats",
     "dex_host_kematrix", "dex_host_getinverse", "dex_host_solve_gevp",
+    "dex_write_eigvecs", "dex_read_eigvecs_info", "dex_read_eigvecs", "dex_host_sixd_fix",
 ]
 
 
@@ -120,6 +121,10 @@ def load() -> C.CDLL:
         "dex_get_kernel_stats": (C.c_int, [P, C.c_int32, dp, C.POINTER(C.c_int64), dp, dp]),
         "dex_reset_kernel_stats": (C.c_int, [P]),
         "dex_host_kematrix": (C.c_int, [C.c_int32, C.c_double, dp]),
+        "dex_write_eigvecs": (C.c_int, [C.c_char_p, dp, C.c_int32, C.c_int32]),
+        "dex_read_eigvecs_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "dex_read_eigvecs": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, dp]),
+        "dex_host_sixd_fix": (C.c_int, [dp, C.c_int32, dp, C.c_int32]),
         "dex_host_getinverse": (C.c_int, [C.c_int32, C.c_double, C.c_int32, C.c_int32, dp]),
         "dex_host_solve_gevp": (C.c_int, [C.c_int32, dp, dp, C.c_int32, C.c_int32, dp, dp]),
     }

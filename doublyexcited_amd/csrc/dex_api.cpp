@@ -46,6 +46,10 @@ constexpr int kMaxSmall = 96 * 64;
 
 }  // namespace
 
+namespace dex {
+int set_error(int status, const char *msg) { return fail(status, msg); }
+}  // namespace dex
+
 struct dex_ctx {
   dex_params p{};
   int nn = 0;

@@ -12,6 +12,9 @@
 
 namespace dex {
 
+// Records msg as the calling thread's last error and returns status.
+int set_error(int status, const char *msg);
+
 // ---- host set-up (dex_setup.cpp) -----------------------------------------
 double kinetic_tau(int m);
 void kematrix(int nn, double delta, double *t);

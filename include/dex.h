@@ -213,6 +213,21 @@ int dex_diagsubspace_device(dex_ctx *ctx, const double *orbitals, int32_t numorb
 int64_t dex_gplmr_footprint(const dex_ctx *ctx, int32_t nev, int32_t ns, int32_t inner_iter,
                             const dex_solve_opts *opts);
 
+/* Eigenvector files in the reference's format, READWRITE_EIGENVECTORS
+ * (utils.F90:82-181): a Fortran unformatted sequential file (gfortran record
+ * framing, subrecords beyond 2147483639 bytes) with a header record M, NUM,
+ * SIZEOF(vector) and one M x M complex128 record per vector.  The size field
+ * is written as int64 and read as int64 or, from a 12-byte header, as its low
+ * 32 bits (the reference reads it into a 4-byte integer, utils.F90:95, which
+ * wraps at 2 GiB).  Reading into a different odd-cubed M applies SIXD_FIX
+ * (utils.F90:184-198): the common central block is kept, the rest is zero.
+ * psi: host, num consecutive vectors of M*M complex (interleaved doubles). */
+int dex_write_eigvecs(const char *path, const double *psi, int32_t m, int32_t num);
+int dex_read_eigvecs_info(const char *path, int32_t *m, int32_t *num);
+int dex_read_eigvecs(const char *path, int32_t m, int32_t num, double *psi);
+/* SIXD_FIX alone: one vector from the (2 n_in + 1)^6 grid to the (2 n_out + 1)^6 grid. */
+int dex_host_sixd_fix(const double *in, int32_t n_in, double *out, int32_t n_out);
+
 /* Host-only set-up and small-matrix routines (no device use), exported so the
  * host arithmetic can be checked on machines without a GPU:
  *   KEMATRIX (hammod.F90:235-260): t = NN x NN column-major, real;
