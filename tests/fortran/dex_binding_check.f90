@@ -9,7 +9,7 @@ module dexmod
   type, bind(C) :: dex_solve_opts
      integer(c_int32_t) :: strict_cholesky, verbose, refresh_every, fp64_restarts
      real(c_double)     :: fp32_tol
-     integer(c_int32_t) :: symmetry
+     integer(c_int32_t) :: symmetry, lock, inner_block, guard, inner_fp32
   end type
   type, bind(C) :: dex_result
      integer(c_int32_t) :: iterations, converged

@@ -117,3 +117,48 @@ def test_host_pencil_degenerate_clusters(seed):
     w = sla.eig(kp, mp, right=False)
     ref = np.sort(np.abs(w[np.argsort(np.abs(w))[:nev]]))
     np.testing.assert_allclose(np.sort(np.abs(mu)), ref, rtol=1e-7)
+
+
+def _c_struct_fields(name):
+    import re
+    from pathlib import Path
+
+    hdr = (Path(__file__).resolve().parent.parent / "include" / "dex.h").read_text()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), hdr, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names = decl.split(None, 1)[1]
+        fields += [n.strip() for n in names.split(",")]
+    return fields
+
+
+def _fortran_type_fields(text, name):
+    import re
+
+    body = re.search(r"type, bind\(C\) :: %s\n(.*?)end type" % name, text, re.S).group(1)
+    fields = []
+    for ln in body.splitlines():
+        if "::" in ln:
+            fields += [n.strip() for n in ln.split("::")[1].split(",")]
+    return fields
+
+
+def test_struct_mirrors_match_the_header():
+    """ctypes mirrors and the Fortran ISO_C_BINDING types (INTEGRATION.md and
+    the compiled Fortran check) list the header's fields in the header's order:
+    a missing trailing field makes dex_solve_opts_default write past the
+    caller's object."""
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parent.parent
+    for cname, pyname in (("dex_solve_opts", "DexSolveOpts"), ("dex_result", "DexResult"),
+                          ("dex_params", "DexParams")):
+        fields = _c_struct_fields(cname)
+        assert [f[0] for f in getattr(_lib, pyname)._fields_] == fields, cname
+        for text in ((root / "INTEGRATION.md").read_text(),
+                     (root / "tests" / "fortran" / "dex_binding_check.f90").read_text()):
+            assert _fortran_type_fields(text, cname) == fields, cname
