@@ -194,6 +194,39 @@ def cpu_baseline(w: dict, matvecs_per_solve: float, pairs_per_solve: float, budg
     }
 
 
+def traffic_run(dex, torch, w: dict, out_path: str) -> None:
+    """The program the HBM-traffic counter passes profile: a calibration kernel
+    of known size (an elementwise multiply reading and writing 1 GiB), then one
+    solve of the workload with the library's per-class launch statistics
+    (algorithmic bytes per launch) written to ``out_path``."""
+    n = 64 * 1024 * 1024
+    a = torch.full((n,), 1.0 + 0.5j, dtype=torch.complex128, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
+    torch.cuda.empty_cache()
+    slot = Slot(dex, w, "fp64")
+    x = torch.empty(slot.g.d * w["nev"], dtype=torch.complex128, device="cuda")
+    slot.g.fill_gaussian_device(x, w["nev"], seed=w["seed"], symmetry=1)
+    r = solve(torch, slot, w, x, True)
+    with open(out_path, "w") as fh:
+        json.dump({k: list(v) for k, v in r["stats"].items()}, fh)
+    slot.g.close()
+    log(f"traffic run: {r['iters']} iterations, {r['conv']} pairs")
+
+
+def traffic_from_profile(w_name: str, kernel: str, alg_bytes_per_launch: float):
+    """HBM bytes per launch of `kernel` from the committed counter measurement
+    of this workload (traffic / algorithmic ratio x this run's algorithmic bytes)."""
+    path = ROOT / "profiles" / f"hbm_traffic_{w_name.replace('he-', '').replace('-', '_')}.json"
+    if not path.exists():
+        return None
+    ratio = json.loads(path.read_text())["traffic_over_algorithmic"].get(kernel)
+    return None if ratio is None else ratio * alg_bytes_per_launch
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +239,8 @@ def main() -> int:
     ap.add_argument("--mixed-inner-fp32", action="store_true",
                     help="mixed form: complex64 Arnoldi basis in the inner solve as well")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-run", metavar="STATS_JSON",
+                    help="only run the program the HBM-traffic counter passes profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: independent solves "
@@ -217,6 +252,10 @@ def main() -> int:
     import doublyexcited_amd as dex
 
     w = WORKLOADS[args.workload]
+    if args.traffic_run:
+        torch.cuda.set_device(0)
+        traffic_run(dex, torch, w, args.traffic_run)
+        return 0
     conc = args.concurrency or w["concurrency"]
     conc = max(1, min(conc, args.steps))
     torch.cuda.set_device(0)
@@ -273,6 +312,10 @@ def main() -> int:
             "ratio_vs_fp64": (pm / wall_mx) / result["value"] if result["value"] > 0 and wall_mx > 0 else None,
         }
     result["roofline"] = algorithmic_roofline(res64)
+    if result["roofline"].get("kernel"):
+        rf = result["roofline"]
+        rf["traffic"] = traffic_from_profile(args.workload, rf["kernel"], rf["alg_bytes_per_launch"])
+        rf["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
     if not args.no_cpu_baseline:
         try:
             result["cpu_baseline"] = cpu_baseline(w, float(np.mean([r["matvecs"] for r in res64])),
