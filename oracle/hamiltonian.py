@@ -21,7 +21,11 @@ Pinning (each is a test under ``tests/``):
     (CSCALE-WORK/Outs/Out.S.N5:51-64) to ~1e-14.
   * ``potential2`` / ``hammult``: a dense Kronecker-product Hamiltonian built
     independently at NN=3 and NN=5, the exchange symmetry P H P = H and
-    complex symmetry H^T = H.
+    complex symmetry H^T = H; and, against the reference itself, the
+    orbital-product subspace eigenvalues and the chosen shift of its logged
+    singlet run (Out.S.N5:97-113, through ``startvec.diagsubspace``) to the
+    accuracy of the log's Lanczos orbitals, which involves the whole
+    two-electron operator including the e-e term.
 """
 from __future__ import annotations
 
