@@ -82,6 +82,10 @@ struct dex_ctx {
   int64_t k_n[DEX_KCLASSES] = {};
   double k_bytes[DEX_KCLASSES] = {};  // algorithmic HBM bytes of the timed launches
   double k_flops[DEX_KCLASSES] = {};  // algorithmic flops of the timed launches
+  // pinned staging ring for the small host-to-device uploads (descriptors,
+  // coefficients): copies from pageable memory would wait for the stream
+  char *h_stage = nullptr;
+  size_t stage_cap = 0, stage_off = 0;
 };
 
 namespace {
@@ -138,6 +142,24 @@ struct KTimer {
     if (c->pending.size() >= 8192) (void)resolve_timing(c);
   }
 };
+
+// Uploads `bytes` from host memory to `dev` on the context's stream through the
+// pinned ring (asynchronous); the ring wraps after a stream synchronisation,
+// so no staged copy is overwritten before it has run.
+hipError_t stage_upload(dex_ctx *c, void *dev, const void *src, size_t bytes) {
+  if (!c->h_stage || bytes > c->stage_cap)
+    return hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, c->stream);
+  const size_t need = (bytes + 255) / 256 * 256;
+  if (c->stage_off + need > c->stage_cap) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    c->stage_off = 0;
+  }
+  char *h = c->h_stage + c->stage_off;
+  c->stage_off += need;
+  std::memcpy(h, src, bytes);
+  return hipMemcpyAsync(dev, h, bytes, hipMemcpyHostToDevice, c->stream);
+}
 
 int ensure_u(dex_ctx *c) {
   if (c->d_u) return 0;
@@ -219,8 +241,7 @@ ColDesc shifted(const Dbl2 *a, const Dbl2 *b, cd sig, int fmt = 0) {
 
 int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
   if (int(ds.size()) > kMaxDesc) return fail(DEX_EINVAL, "too many block columns");
-  DEX_HIP(hipMemcpyAsync(c->d_desc, ds.data(), ds.size() * sizeof(ColDesc), hipMemcpyHostToDevice,
-                         c->stream));
+  DEX_HIP(stage_upload(c, c->d_desc, ds.data(), ds.size() * sizeof(ColDesc)));
   return 0;
 }
 
@@ -310,11 +331,9 @@ int combine(dex_ctx *c, const std::vector<ColDesc> &in, const std::vector<Dbl2 *
   std::vector<ColDesc> ds(in);
   for (Dbl2 *o : out) ds.push_back(plain(o, out_fmt));
   if (int st = upload_descs(c, ds)) return st;
-  DEX_HIP(hipMemcpyAsync(c->d_small, cm.data(), cm.size() * sizeof(Dbl2), hipMemcpyHostToDevice,
-                         c->stream));
+  DEX_HIP(stage_upload(c, c->d_small, cm.data(), cm.size() * sizeof(Dbl2)));
   if (beta) {
-    DEX_HIP(hipMemcpyAsync(c->d_small2, beta->data(), beta->size() * sizeof(Dbl2),
-                           hipMemcpyHostToDevice, c->stream));
+    DEX_HIP(stage_upload(c, c->d_small2, beta->data(), beta->size() * sizeof(Dbl2)));
   }
   KTimer kt(c, DEX_K_COMBINE,
             16.0 * double(c->d) * (distinct_vectors(in, in.size()) + double(nout) * (beta ? 2.0 : 1.0)),
@@ -469,6 +488,10 @@ int dex_ctx_create(const dex_params *p, dex_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&c->d_partial, c->partial_bytes);
   if (e == hipSuccess) e = hipMalloc(&c->d_dots, 64 * 4 * sizeof(double));
   for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+  if (e == hipSuccess) {
+    c->stage_cap = size_t(4) << 20;
+    e = hipHostMalloc(reinterpret_cast<void **>(&c->h_stage), c->stage_cap, hipHostMallocDefault);
+  }
   if (e != hipSuccess) {
     dex_ctx_destroy(c);
     return hip_fail(e, "dex_ctx_create");
@@ -492,6 +515,7 @@ int dex_ctx_destroy(dex_ctx *c) {
     (void)hipEventDestroy(pd.b);
   }
   for (hipEvent_t ev : c->free_events) (void)hipEventDestroy(ev);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DEX_OK;
