@@ -60,6 +60,14 @@ WORKLOADS = {
                             seed=1002, ns=1, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
                             desc="He, NN=21 (D=85.8M), delta 0.4, theta 0.35, block 16, "
                                  "sigma -0.70-0.001i, tol 1e-9, inner solve order 4 on 4-column groups"),
+    # theta-trajectory scan: theta evenly spaced over [0.20, 0.50], NN cycling 11, 13, 15, 17;
+    # each point warm-started from the previous one (SIXD_FIX across grids); a step is one point
+    "he-theta-scan": dict(n=5, delta=0.5, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-8,
+                          seed=1004, ns=3, inner=8, inner_block=0, guard=2, maxit=800, concurrency=1,
+                          scan=dict(points=64, theta=(0.20, 0.50), nn=(11, 13, 15, 17)),
+                          desc="He theta scan: 64 thetas evenly spaced over [0.20, 0.50], NN cycling "
+                               "11/13/15/17, delta 0.5, 8 pairs nearest -0.78-0.002i (block 8 + 2 guard), "
+                               "tol 1e-8, NS 3, inner order 8, warm starts"),
     # the mixed-precision configuration (FP64 basis: 2 restarts after the FP32 phase)
     "he-nn25-block8": dict(n=12, delta=0.35, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-7,
                            seed=1003, ns=1, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
@@ -227,6 +235,42 @@ def traffic_from_profile(w_name: str, kernel: str, alg_bytes_per_launch: float):
     return None if ratio is None else ratio * alg_bytes_per_launch
 
 
+def run_scan(w: dict, args) -> int:
+    """The theta-scan workload: the first --steps points of the scan, in order
+    (warm starts), timed as a whole."""
+    import numpy as np
+
+    from doublyexcited_amd.scan import theta_scan
+
+    sc = w["scan"]
+    th = np.linspace(sc["theta"][0], sc["theta"][1], sc["points"])
+    pts = [((sc["nn"][i % len(sc["nn"])] - 1) // 2, w["delta"], float(th[i])) for i in range(sc["points"])]
+    pts = pts[:args.steps]
+    kw = dict(sigma=w["sigma"], nev=w["nev"], tol=w["tol"], ns=w["ns"], inner_iter=w["inner"],
+              guard=w["guard"], maxit=w["maxit"], seed=w["seed"])
+    if args.warmup:
+        theta_scan(pts[:1], warm=False, **kw)
+    t0 = time.time()
+    res = theta_scan(pts, warm=True, **kw)
+    wall = time.time() - t0
+    pairs = sum(p.converged_pairs for p in res)
+    lat = [p.seconds for p in res]
+    for p in res:
+        log(f"  NN={2 * p.n + 1} theta={p.theta:.4f}: {p.seconds:.2f}s iters {p.iterations} "
+            f"converged {p.converged_pairs}/{w['nev']} warm {p.warm}")
+    print(json.dumps({
+        "metric": "resonance eigenpairs/sec + p99 solve latency; mixed vs FP64 GPU at 1e-7 au",
+        "value": pairs / wall if wall > 0 else 0.0, "unit": "eigenpairs/s", "n_gpus": 1,
+        "steps": len(pts), "warmup": args.warmup, "ms_per_step": wall / len(pts) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "complex128",
+        "data": "synthetic: seeded Philox complex-Gaussian singlet start block for the first point",
+        "config": {"workload": args.workload, "desc": w["desc"], "points": len(pts), "parallelism": "1 GPU"},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "converged_pairs": pairs, "iterations": [p.iterations for p in res],
+    }))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +300,9 @@ def main() -> int:
         torch.cuda.set_device(0)
         traffic_run(dex, torch, w, args.traffic_run)
         return 0
+    if "scan" in w:
+        torch.cuda.set_device(0)
+        return run_scan(w, args)
     conc = args.concurrency or w["concurrency"]
     conc = max(1, min(conc, args.steps))
     torch.cuda.set_device(0)
