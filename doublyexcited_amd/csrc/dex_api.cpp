@@ -305,11 +305,22 @@ int gram_self(dex_ctx *c, const std::vector<ColDesc> &x, std::vector<cd> &g) {
   return gram_launch_copy(c, x, k, k, k, g, k, 0, 0);
 }
 
-// [K | M] = B^H [B | Q] with B = AQ - sigma Q (nq <= 64): AQ and Q are read once.
+// [K | M] = B^H [B | Q] with B = AQ - sigma Q: for nq <= 64 one launch that reads
+// AQ and Q once; wider bases (up to 96) go through the chunked general Gram.
+int gram(dex_ctx *c, const std::vector<ColDesc> &l, const std::vector<ColDesc> &r, std::vector<cd> &g);
+
 int gram_harmonic(dex_ctx *c, const std::vector<Dbl2 *> &q, const std::vector<Dbl2 *> &aq, cd sigma,
                   std::vector<cd> &g) {
   const int nq = int(q.size());
-  if (nq < 1 || nq > 64) return fail(DEX_EINVAL, "projected basis must have 1..64 columns");
+  if (nq < 1 || nq > 96) return fail(DEX_EINVAL, "projected basis must have 1..96 columns");
+  if (nq > 64) {
+    // beyond one launch: the general chunked Gram of B = AQ - sigma Q against [B | Q]
+    std::vector<ColDesc> l, r;
+    for (int j = 0; j < nq; ++j) l.push_back(shifted(aq[j], q[j], sigma));
+    r = l;
+    for (int j = 0; j < nq; ++j) r.push_back(plain(q[j]));
+    return gram(c, l, r, g);
+  }
   const int ma = (nq + 15) / 16;
   std::vector<ColDesc> ds;
   for (int j = 0; j < nq; ++j) {
@@ -1303,7 +1314,15 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     std::vector<Dbl2 *> out = X.cols;
     out.insert(out.end(), AX.cols.begin(), AX.cols.end());
     out.insert(out.end(), W.cols.begin(), W.cols.end());
-    if (int st = combine(c, in, out, cm)) return st;
+    if (3 * nev <= 48) {
+      if (int st = combine(c, in, out, cm)) return st;
+    } else {
+      // more outputs than one launch takes: W first, then X and AX in place
+      const std::vector<cd> cw(cm.begin() + size_t(ld) * 2 * nev, cm.end());
+      const std::vector<cd> cx(cm.begin(), cm.begin() + size_t(ld) * 2 * nev);
+      if (int st = combine(c, in, W.cols, cw)) return st;
+      if (int st = combine(c, in, std::vector<Dbl2 *>(out.begin(), out.begin() + 2 * nev), cx)) return st;
+    }
     S.t_update += now_s() - t0;
     t0 = now_s();
     if (int st = coldots(c, descs_of(W), {}, aa, ab, bb)) return st;
@@ -1402,7 +1421,7 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     for (const Block *b : {&AX, &AW, &ASblk, &AP})
       AQ.cols.insert(AQ.cols.end(), b->cols.begin(), b->cols.end());
     const int nq = Q.size();
-    if (nq > 64) return fail(DEX_EINVAL, "projected basis larger than 64");
+    if (nq > 96) return fail(DEX_EINVAL, "projected basis larger than 96");
     std::vector<cd> km;
     double t0 = now_s();
     if (int st = gram_harmonic(c, Q.cols, AQ.cols, sigma, km)) return st;
@@ -1423,9 +1442,12 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     // The new P keeps the columns of the active set.
     const int nw = W.size(), nsb = Sblk.size(), np = P.size();
     const int nact = int(active.size());
-    // inputs [X W S P], outputs [X, Pnew]
+    // inputs [X W S P], outputs [X, Pnew].  One launch takes at most 64
+    // inputs; beyond that the inputs go in chunks (X, written in place, in the
+    // first) and the new P gets fresh columns, since the old P is read later.
+    const bool chunked = nq > 64;
     Block Pn, APn;
-    for (int j = 0; j < np && j < nact; ++j) {
+    for (int j = 0; !chunked && j < np && j < nact; ++j) {
       Pn.cols.push_back(P.cols[j]);
       APn.cols.push_back(AP.cols[j]);
     }
@@ -1439,16 +1461,30 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
       for (int i = nev; i < nq; ++i) cm[i + size_t(nq) * (nev + jj)] = coords[i + size_t(nq) * j];  // P_new
     }
     {
-      std::vector<Dbl2 *> out = X.cols;
+      // one pass per input chunk over every output: nev + nact <= 2 (NEV + guard)
+      // <= 48, since (3 + NS)(NEV + guard) <= 96
+      std::vector<Dbl2 *> out = X.cols, aout = AX.cols;
       out.insert(out.end(), Pn.cols.begin(), Pn.cols.end());
-      if (int st = combine(c, descs_of(Q), out, cm)) return st;
-      std::vector<Dbl2 *> aout = AX.cols;
       aout.insert(aout.end(), APn.cols.begin(), APn.cols.end());
-      if (int st = combine(c, descs_of(AQ), aout, cm)) return st;
+      const int nout = int(out.size());
+      std::vector<cd> one(nout, 1.0);
+      for (int i0 = 0; i0 < nq; i0 += 64) {
+        const int bi = std::min(64, nq - i0);
+        std::vector<cd> sub(size_t(bi) * nout);
+        for (int j = 0; j < nout; ++j)
+          for (int i = 0; i < bi; ++i) sub[i + size_t(bi) * j] = cm[i0 + i + size_t(nq) * j];
+        std::vector<ColDesc> qd, aqd;
+        for (int i = 0; i < bi; ++i) {
+          qd.push_back(plain(Q.cols[i0 + i]));
+          aqd.push_back(plain(AQ.cols[i0 + i]));
+        }
+        if (int st = combine(c, qd, out, sub, i0 > 0 ? &one : nullptr)) return st;
+        if (int st = combine(c, aqd, aout, sub, i0 > 0 ? &one : nullptr)) return st;
+      }
     }
     (void)nw;
     (void)nsb;
-    for (int j = nact; j < np; ++j) {
+    for (int j = chunked ? 0 : nact; j < np; ++j) {
       S.pool.push_back(P.cols[j]);
       S.pool.push_back(AP.cols[j]);
     }
@@ -1503,7 +1539,8 @@ extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns
   const int64_t ib = opts ? opts->inner_block : 0;
   const int64_t grp = ib > 0 && ib < nblk ? ib : nblk;
   const int64_t inner = inner_iter >= 2 ? int64_t(inner_iter + 2) * grp : (inner_iter == 1 ? nblk : 0);
-  return (int64_t(7 + 2 * ns) * nblk + inner) * vec + vec;
+  const int64_t fresh_p = (3 + ns) * nblk > 64 ? 2 * nblk : 0;  // the chunked update's new P, AP
+  return (int64_t(7 + 2 * ns) * nblk + inner + fresh_p) * vec + vec;
 }
 
 extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,
@@ -1516,8 +1553,8 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
     return fail(DEX_EUNSUPPORTED, "inner_fp32 needs the FP32 matvec kernels for this grid");
   const int guard = opts ? opts->guard : 0;
   if (guard < 0) return fail(DEX_EINVAL, "guard must be >= 0");
-  if (nev < 1 || ns < 1 || (3 + ns) * (nev + guard) > 64)
-    return fail(DEX_EINVAL, "need NEV >= 1, NS >= 1, (3+NS)*(NEV+guard) <= 64");
+  if (nev < 1 || ns < 1 || ns * (nev + guard) > 48 || (3 + ns) * (nev + guard) > 96)
+    return fail(DEX_EINVAL, "need NEV >= 1, NS >= 1, NS*(NEV+guard) <= 48, (3+NS)*(NEV+guard) <= 96");
   if (inner_iter < 0) return fail(DEX_EINVAL, "GM_MAX_INNER_ITER must be >= 0");
   if (opts && (opts->symmetry < -1 || opts->symmetry > 1)) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
   if (opts && opts->symmetry != 0 && !c->fast64 && !(c->p.precision == DEX_FP32 && c->fast32))

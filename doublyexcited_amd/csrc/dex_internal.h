@@ -89,7 +89,7 @@ hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t 
 // out_j = beta_j out_j + sum_i in_i C(i, j) row by row; in-place aliasing of
 // an output with an input is allowed.  `descs`: nin input descriptors then
 // nout output pointers (as ColDesc.a); C (nin x nout) and beta (nout, may be
-// null) are device arrays.
+// null) are device arrays.  nin <= 64, nout <= 48.
 hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c,
                           const Dbl2 *beta, int64_t d, hipStream_t s);
 

@@ -749,6 +749,7 @@ constexpr int kCombRows = 32;
 constexpr int kCombThreads = 256;
 constexpr int kCombMaxIn = 64;
 constexpr int kCombGroups = 8;  // output groups per workgroup (kCombThreads / kCombRows)
+constexpr int kCombMaxOut = 48;
 
 // out_j = sum_i in_i C(i, j) (+ beta_j out_j), row by row.  A workgroup stages
 // C (nin x nout, padded to kCombGroups * JPT columns) once in LDS, then walks
@@ -862,7 +863,8 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
   }
 }
 
-// The combine for narrow outputs (nout <= 16), without LDS: a wavefront owns
+// The combine for narrow outputs (nout <= 16; also 33..48, where the LDS
+// kernel's staging would exceed 64 KB per workgroup), without LDS: a wavefront owns
 // 64 consecutive rows (one per lane) and every output column of them, so the
 // coefficients C(i, j) are wavefront-uniform (scalar loads, no LDS traffic)
 // and each input element is read once, by one lane, in one coalesced 1 KB
@@ -1191,8 +1193,8 @@ hipError_t combine_launch(const ColDesc *descs, int nin, int nout, const Dbl2 *c
 
 hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
                           int64_t d, hipStream_t s) {
-  if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombGroups * 6) return hipErrorInvalidValue;
-  if (nout <= 16) {
+  if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombMaxOut) return hipErrorInvalidValue;
+  if (nout <= 16 || nout > kCombGroups * 4) {
     const int64_t waves = (d + 63) / 64;
     const unsigned grid = unsigned((waves + 3) / 4 < 2048 ? (waves + 3) / 4 : 2048);
     const double2 *cc = reinterpret_cast<const double2 *>(c);
@@ -1201,17 +1203,16 @@ hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c
       hipLaunchKernelGGL(k_combine_w<4>, dim3(grid), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
     else if (nout <= 8)
       hipLaunchKernelGGL(k_combine_w<8>, dim3(grid), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
-    else
+    else if (nout <= 16)
       hipLaunchKernelGGL(k_combine_w<16>, dim3(grid), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
+    else
+      hipLaunchKernelGGL(k_combine_w<kCombMaxOut>, dim3(grid), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
     return hipGetLastError();
   }
+  // 17..32 outputs: C and one row tile in LDS, (64 * 8 * 4 + 32 * 64) * 16 B = 64 KB at most
   switch ((nout + kCombGroups - 1) / kCombGroups) {
-    case 1: return combine_launch<1>(descs, nin, nout, c, beta, d, s);
-    case 2: return combine_launch<2>(descs, nin, nout, c, beta, d, s);
     case 3: return combine_launch<3>(descs, nin, nout, c, beta, d, s);
-    case 4: return combine_launch<4>(descs, nin, nout, c, beta, d, s);
-    case 5: return combine_launch<5>(descs, nin, nout, c, beta, d, s);
-    default: return combine_launch<6>(descs, nin, nout, c, beta, d, s);
+    default: return combine_launch<4>(descs, nin, nout, c, beta, d, s);
   }
 }
 

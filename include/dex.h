@@ -142,7 +142,8 @@ typedef struct dex_solve_opts {
                               caller's NEV: convergence is tested, and values returned, on
                               the NEV nearest; a block that would end inside a degenerate
                               level then holds the whole level.  0 (default): the
-                              reference's block of exactly NEV.  Needs (3+NS)(NEV+guard) <= 64 */
+                              reference's block of exactly NEV.  Needs NS(NEV+guard) <= 48 and
+                              (3+NS)(NEV+guard) <= 96 */
   int32_t inner_fp32;      /* inner solve: 1 keeps its Arnoldi basis in complex64 and applies H
                               in FP32 (half the traffic of the preconditioner; the outer
                               iteration, its residuals and the returned pairs stay FP64);

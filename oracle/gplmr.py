@@ -202,10 +202,11 @@ class GplmrResult:
 def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: complex,
               nev: int, ns: int = 1, maxit: int = 100, tol: float = 1e-8,
               inner_iter: int = 0, callback=None, lock: bool = False) -> GplmrResult:
-    """GPLMR_EIG, gplmr.F90:9-386, with GM_MAX_INNER_ITER in {0, 1}.
+    """GPLMR_EIG, gplmr.F90:9-386.
 
     ``inner_iter == 1`` applies PRECBLOCK's ``W <- H W`` (gplmr.F90:491-493);
-    ``0`` skips the preconditioner (guards at gplmr.F90:92,118,151).
+    ``0`` skips the preconditioner (guards at gplmr.F90:92,118,151); ``>= 2``
+    runs PRECBLOCK_BASIC's block solve (``basic_block_lan_solve``).
 
     ``lock`` (not in the reference): search directions W, S and P are built
     only for columns whose residual is still above ``tol``; every column stays

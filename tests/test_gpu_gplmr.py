@@ -164,6 +164,46 @@ def test_guard_columns_return_nearest_pairs(dex):
         assert np.linalg.norm(h.hammult(xj, v) - rg.e[j] * xj) <= 10 * 1e-10 * np.linalg.norm(xj)
 
 
+@pytest.mark.parametrize("nev,ns", [(12, 3), (16, 2)])
+def test_wide_projection_trajectory(dex, nev, ns):
+    """Projected bases wider than one Gram / combine launch takes (72 and 80
+    columns > 64): the harmonic Gram and the update go in chunks.  Step-by-step
+    parity with the oracle over 8 iterations, as in the trajectory test."""
+    print(f"case: wide projection nev={nev} ns={ns}", flush=True)
+    h = Hamiltonian(2, 0.5, 0.3)
+    v = h.potential2()
+    sigma = -0.78 - 0.002j
+    x0 = start_block(h, nev, 11)
+    its = 8
+    ref = gplmr_eig(lambda b: np.stack([h.hammult(b[:, j], v) for j in range(b.shape[1])], axis=1),
+                    x0, sigma, nev, ns=ns, maxit=its, tol=1e-14)
+    x = x0.copy(order="F")
+    with dex.HeliumGrid(2, 0.5, 0.3) as g:
+        r = g.gplmr_eig(x, sigma, nev, ns=ns, maxit=its, tol=1e-14)
+    assert r.iterations == ref.iters == its
+    assert match(r.e, ref.e) < 1e-8 and match(ref.e, r.e) < 1e-8
+    assert r.max_residual == pytest.approx(float(ref.resnrm.max()), rel=1e-4)
+
+
+def test_wide_projection_guard_converges(dex):
+    """NEV=16 with 2 guard columns and NS=2 (90 projected columns): converges,
+    every returned pair meets the tolerance against the oracle's operator and
+    the values come nearest-first."""
+    print("case: wide projection guard nev=16", flush=True)
+    h = Hamiltonian(2, 0.5, 0.3)
+    v = h.potential2()
+    nev, sigma, tol = 16, -0.78 - 0.002j, 1e-9
+    x = start_block(h, nev, 5).copy(order="F")
+    with dex.HeliumGrid(2, 0.5, 0.3) as g:
+        r = g.gplmr_eig(x, sigma, nev, ns=2, maxit=400, tol=tol, guard=2)
+    assert r.converged
+    dist = np.abs(r.e - sigma)
+    assert np.all(np.diff(dist) >= -1e-9)
+    for j in range(nev):
+        xj = x[:, j]
+        assert np.linalg.norm(h.hammult(xj, v) - r.e[j] * xj) <= 10 * tol * np.linalg.norm(xj)
+
+
 def test_fp32_inner_solve_same_pairs(dex):
     """The inner solve with a complex64 Arnoldi basis and FP32 matvecs is only a
     different preconditioner: the outer FP64 iteration returns the same pairs
