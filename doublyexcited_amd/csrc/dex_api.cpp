@@ -1442,12 +1442,9 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     // The new P keeps the columns of the active set.
     const int nw = W.size(), nsb = Sblk.size(), np = P.size();
     const int nact = int(active.size());
-    // inputs [X W S P], outputs [X, Pnew].  One launch takes at most 64
-    // inputs; beyond that the inputs go in chunks (X, written in place, in the
-    // first) and the new P gets fresh columns, since the old P is read later.
-    const bool chunked = nq > 64;
+    // inputs [X W S P], outputs [X, Pnew] with Pnew in the old P's storage.
     Block Pn, APn;
-    for (int j = 0; !chunked && j < np && j < nact; ++j) {
+    for (int j = 0; j < np && j < nact; ++j) {
       Pn.cols.push_back(P.cols[j]);
       APn.cols.push_back(AP.cols[j]);
     }
@@ -1461,8 +1458,14 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
       for (int i = nev; i < nq; ++i) cm[i + size_t(nq) * (nev + jj)] = coords[i + size_t(nq) * j];  // P_new
     }
     {
-      // one pass per input chunk over every output: nev + nact <= 2 (NEV + guard)
-      // <= 48, since (3 + NS)(NEV + guard) <= 96
+      // One launch takes at most 64 inputs and 48 outputs (nev + nact <=
+      // 2 (NEV + guard) <= 48 since (3 + NS)(NEV + guard) <= 96).  Wider Q goes
+      // in chunks accumulated with beta = 1, X and P (whose storage the outputs
+      // overwrite: 2 (NEV + guard) <= 48 columns) in the first, W and S after.
+      std::vector<int> order;
+      for (int i = 0; i < nev; ++i) order.push_back(i);
+      for (int i = nev + nw + nsb; i < nq; ++i) order.push_back(i);
+      for (int i = nev; i < nev + nw + nsb; ++i) order.push_back(i);
       std::vector<Dbl2 *> out = X.cols, aout = AX.cols;
       out.insert(out.end(), Pn.cols.begin(), Pn.cols.end());
       aout.insert(aout.end(), APn.cols.begin(), APn.cols.end());
@@ -1472,19 +1475,17 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
         const int bi = std::min(64, nq - i0);
         std::vector<cd> sub(size_t(bi) * nout);
         for (int j = 0; j < nout; ++j)
-          for (int i = 0; i < bi; ++i) sub[i + size_t(bi) * j] = cm[i0 + i + size_t(nq) * j];
+          for (int i = 0; i < bi; ++i) sub[i + size_t(bi) * j] = cm[order[i0 + i] + size_t(nq) * j];
         std::vector<ColDesc> qd, aqd;
         for (int i = 0; i < bi; ++i) {
-          qd.push_back(plain(Q.cols[i0 + i]));
-          aqd.push_back(plain(AQ.cols[i0 + i]));
+          qd.push_back(plain(Q.cols[order[i0 + i]]));
+          aqd.push_back(plain(AQ.cols[order[i0 + i]]));
         }
         if (int st = combine(c, qd, out, sub, i0 > 0 ? &one : nullptr)) return st;
         if (int st = combine(c, aqd, aout, sub, i0 > 0 ? &one : nullptr)) return st;
       }
     }
-    (void)nw;
-    (void)nsb;
-    for (int j = chunked ? 0 : nact; j < np; ++j) {
+    for (int j = nact; j < np; ++j) {
       S.pool.push_back(P.cols[j]);
       S.pool.push_back(AP.cols[j]);
     }
@@ -1539,8 +1540,7 @@ extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns
   const int64_t ib = opts ? opts->inner_block : 0;
   const int64_t grp = ib > 0 && ib < nblk ? ib : nblk;
   const int64_t inner = inner_iter >= 2 ? int64_t(inner_iter + 2) * grp : (inner_iter == 1 ? nblk : 0);
-  const int64_t fresh_p = (3 + ns) * nblk > 64 ? 2 * nblk : 0;  // the chunked update's new P, AP
-  return (int64_t(7 + 2 * ns) * nblk + inner + fresh_p) * vec + vec;
+  return (int64_t(7 + 2 * ns) * nblk + inner) * vec + vec;
 }
 
 extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,
