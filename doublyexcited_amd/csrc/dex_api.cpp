@@ -61,13 +61,13 @@ struct dex_ctx {
   bool fast64 = false, fast32 = false;
   void *d_u = nullptr;  // matvec intermediate, D complex128 (also used as D complex64)
   ColDesc *d_desc = nullptr;
-  Dbl2 *d_small = nullptr;
-  Dbl2 *d_small2 = nullptr;
+  unsigned char *d_args = nullptr;  // a combine's descriptors, C and beta, packed (one upload)
   Dbl2 *d_gram_out = nullptr;
   void *d_partial = nullptr;
   size_t partial_bytes = 0;
   double *d_dots = nullptr;
-  bool timing = false;
+  int timing = 0;                 // 0 off, 1 every launch, n > 1 a random 1 in n of them
+  uint64_t sample_state = 0x9e3779b97f4a7c15ull;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double t_pass1 = -1.0, t_pass2 = -1.0;
   // per-kernel-class launch timing: event pairs recorded on the stream and
@@ -119,12 +119,25 @@ hipError_t resolve_timing(dex_ctx *c) {
 
 // Brackets the launches of one kernel class with events when timing is on.
 // `bytes` and `flops` are the launch's algorithmic HBM traffic and arithmetic.
+// Whether this launch is timed: all with timing 1; with timing n > 1 each
+// launch independently with probability 1/n (xorshift64), so a periodic
+// launch pattern cannot alias with the sample.
+bool sampled(dex_ctx *c) {
+  if (c->timing <= 1) return c->timing == 1;
+  uint64_t x = c->sample_state;
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  c->sample_state = x;
+  return (x >> 11) % uint64_t(c->timing) == 0;
+}
+
 struct KTimer {
   dex_ctx *c;
   int cls;
   hipEvent_t a = nullptr;
   KTimer(dex_ctx *c_, int cls_, double bytes, double flops) : c(c_), cls(cls_) {
-    if (c->timing && (a = take_event(c)) != nullptr) {
+    if (c->timing && sampled(c) && (a = take_event(c)) != nullptr) {
       (void)hipEventRecord(a, c->stream);
       c->k_bytes[cls] += bytes;
       c->k_flops[cls] += flops;
@@ -177,14 +190,16 @@ hipError_t second_pass(dex_ctx *c, int io, const void *x, void *y, int sym) {
 
 // One application of H on complex128 device vectors; `single` selects FP32
 // arithmetic (complex64 intermediate, complex128 in and out).
-int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0) {
+// pass_events: also record c->ev[0..2] around the two passes (the matvec
+// entry point's per-pass timing).
+int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0, bool pass_events = false) {
   if (int st = ensure_u(c)) return st;
   hipStream_t s = c->stream;
   const bool fast = single ? c->fast32 : c->fast64;
   if (single && !fast) return fail(DEX_EUNSUPPORTED, "no FP32 matvec for this grid");
   if (fast) {
     const int io = single ? dex::kIoMixed : dex::kIoF64;
-    if (c->timing) DEX_HIP(hipEventRecord(c->ev[0], s));
+    if (c->timing && pass_events) DEX_HIP(hipEventRecord(c->ev[0], s));
     const double dd = double(c->d), nn = double(c->nn);
     const double ub = single ? 8.0 : 16.0;  // intermediate U entry bytes
     {
@@ -192,7 +207,7 @@ int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0) {
       KTimer kt(c, DEX_K_PASS1, (16.0 + ub) * dd, 12.0 * nn * dd);
       DEX_HIP(dex::launch_pass1(c->nn, io, x, c->d_u, s));
     }
-    if (c->timing) DEX_HIP(hipEventRecord(c->ev[1], s));
+    if (c->timing && pass_events) DEX_HIP(hipEventRecord(c->ev[1], s));
     {
       // reads U and X, writes Y; the general pass also applies the three
       // electron-2 axes, the transpose-add pass the exchange image
@@ -200,7 +215,7 @@ int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0) {
       KTimer kt(c, sym != 0 ? DEX_K_SYMADD : DEX_K_PASS2, (32.0 + ub) * dd, fl);
       DEX_HIP(second_pass(c, io, x, y, sym));
     }
-    if (c->timing) DEX_HIP(hipEventRecord(c->ev[2], s));
+    if (c->timing && pass_events) DEX_HIP(hipEventRecord(c->ev[2], s));
     return 0;
   }
   KTimer kt(c, DEX_K_GENERIC, 32.0 * double(c->d), (24.0 * c->nn + 14.0) * double(c->d));
@@ -338,18 +353,27 @@ int combine(dex_ctx *c, const std::vector<ColDesc> &in, const std::vector<Dbl2 *
             const std::vector<cd> &cm, const std::vector<cd> *beta = nullptr, int out_fmt = 0) {
   const int nin = int(in.size()), nout = int(out.size());
   if (nout == 0) return 0;
-  if (nin * nout > kMaxSmall) return fail(DEX_EINVAL, "combine matrix too large");
+  if (nin * nout > kMaxSmall || cm.size() < size_t(nin) * nout || cm.size() > size_t(kMaxSmall) ||
+      (beta && (beta->size() < size_t(nout) || beta->size() > size_t(kMaxSmall))))
+    return fail(DEX_EINVAL, "combine matrix size");
   std::vector<ColDesc> ds(in);
   for (Dbl2 *o : out) ds.push_back(plain(o, out_fmt));
-  if (int st = upload_descs(c, ds)) return st;
-  DEX_HIP(stage_upload(c, c->d_small, cm.data(), cm.size() * sizeof(Dbl2)));
-  if (beta) {
-    DEX_HIP(stage_upload(c, c->d_small2, beta->data(), beta->size() * sizeof(Dbl2)));
-  }
+  if (int(ds.size()) > kMaxDesc) return fail(DEX_EINVAL, "too many block columns");
+  // descriptors | C | beta in one image, one copy
+  const size_t bd = (ds.size() * sizeof(ColDesc) + 255) / 256 * 256;
+  const size_t bc = (cm.size() * sizeof(Dbl2) + 255) / 256 * 256;
+  const size_t bb = beta ? beta->size() * sizeof(Dbl2) : 0;
+  std::vector<unsigned char> img(bd + bc + bb);
+  std::memcpy(img.data(), ds.data(), ds.size() * sizeof(ColDesc));
+  std::memcpy(img.data() + bd, cm.data(), cm.size() * sizeof(Dbl2));
+  if (beta) std::memcpy(img.data() + bd + bc, beta->data(), bb);
+  DEX_HIP(stage_upload(c, c->d_args, img.data(), img.size()));
   KTimer kt(c, DEX_K_COMBINE,
             16.0 * double(c->d) * (distinct_vectors(in, in.size()) + double(nout) * (beta ? 2.0 : 1.0)),
             8.0 * double(c->d) * nin * nout);
-  DEX_HIP(dex::launch_combine(c->d_desc, nin, nout, c->d_small, beta ? c->d_small2 : nullptr, c->d,
+  const Dbl2 *dc = reinterpret_cast<const Dbl2 *>(c->d_args + bd);
+  const Dbl2 *dbeta = beta ? reinterpret_cast<const Dbl2 *>(c->d_args + bd + bc) : nullptr;
+  DEX_HIP(dex::launch_combine(reinterpret_cast<const ColDesc *>(c->d_args), nin, nout, dc, dbeta, c->d,
                               c->stream));
   return 0;
 }
@@ -491,8 +515,7 @@ int dex_ctx_create(const dex_params *p, dex_ctx **out) {
   if (e == hipSuccess)
     e = hipMemcpy(c->d_vn, c->vn_host.data(), c->vn_host.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&c->d_desc, kMaxDesc * sizeof(ColDesc));
-  if (e == hipSuccess) e = hipMalloc(&c->d_small, kMaxSmall * sizeof(Dbl2));
-  if (e == hipSuccess) e = hipMalloc(&c->d_small2, kMaxSmall * sizeof(Dbl2));
+  if (e == hipSuccess) e = hipMalloc(&c->d_args, kMaxDesc * sizeof(ColDesc) + 2 * kMaxSmall * sizeof(Dbl2) + 512);
   if (e == hipSuccess) e = hipMalloc(&c->d_gram_out, 192 * 192 * sizeof(Dbl2));
   c->partial_bytes = std::max(dex::gram_partial_elems(64, 128) * sizeof(Dbl2),
                               dex::coldots_partial_elems(64) * sizeof(double));
@@ -516,7 +539,7 @@ int dex_ctx_destroy(dex_ctx *c) {
   (void)hipSetDevice(c->p.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void *ptr : {(void *)c->d_tinv, (void *)c->d_vn, c->d_u, (void *)c->d_desc,
-                    (void *)c->d_small, (void *)c->d_small2, (void *)c->d_gram_out, c->d_partial,
+                    (void *)c->d_args, (void *)c->d_gram_out, c->d_partial,
                     (void *)c->d_dots})
     if (ptr) (void)hipFree(ptr);
   for (auto &ev : c->ev)
@@ -559,7 +582,8 @@ int dex_get_tinv(const dex_ctx *c, double *tinv) {
 
 int dex_set_kernel_timing(dex_ctx *c, int32_t enable) {
   if (!c) return fail(DEX_EINVAL, "null argument");
-  c->timing = enable != 0;
+  if (enable < 0) return fail(DEX_EINVAL, "timing period must be >= 0");
+  c->timing = enable;
   return DEX_OK;
 }
 
@@ -624,7 +648,7 @@ int dex_matvec_sym_device(dex_ctx *c, const void *in, void *out, int32_t howmany
     } else {
       const Dbl2 *xi = static_cast<const Dbl2 *>(in) + size_t(j) * c->d;
       Dbl2 *yo = static_cast<Dbl2 *>(out) + size_t(j) * c->d;
-      if (int st = apply_h(c, xi, yo, c->p.precision == DEX_FP32, symmetry)) return st;
+      if (int st = apply_h(c, xi, yo, c->p.precision == DEX_FP32, symmetry, true)) return st;
     }
   }
   const bool timed_fast = c->p.precision == DEX_FP32 ? c->fast32 : c->fast64;

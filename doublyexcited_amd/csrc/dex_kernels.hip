@@ -15,6 +15,8 @@
 // p map keeps groups of 8 consecutive p on one XCD).  The kinetic operator is
 // real, so the real and imaginary parts are contracted in turn through two
 // LDS buffers of NN^3 reals each (148 KB at NN = 21 in FP64).
+#include <atomic>
+
 #include <hip/hip_runtime.h>
 
 #include "dex_internal.h"
@@ -1143,25 +1145,51 @@ int gram_blocks() { return kGramBlocks; }
 size_t gram_partial_elems(int kl, int kr) { return size_t(kGramBlocks) * kl * kr; }
 
 namespace {
+// Workgroups of k_gram_mfma<MA, NB> that fit on the device at once (LDS sets
+// it: 3 or 4 per CU), capped at kGramBlocks.  A grid of exactly that many runs
+// in one wave of workgroups; 1024 workgroups at 3 per CU would leave a second
+// round on a third of the chip.
 template <int MA, int NB>
-void gram_launch(const ColDesc *descs, int kl, int kr, int share, int64_t d, int64_t rpb,
-                 double2 *partial, hipStream_t s) {
-  hipLaunchKernelGGL((k_gram_mfma<MA, NB>), dim3(kGramBlocks), dim3(kGramThreads), 0, s, descs, kl, kr,
-                     share, d, rpb, partial);
+int gram_grid() {
+  static std::atomic<int> cached{0};
+  int grid = cached.load(std::memory_order_relaxed);
+  if (grid == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_gram_mfma<MA, NB>),
+                                                     kGramThreads, 0) != hipSuccess ||
+        cus < 1 || per_cu < 1)
+      return kGramBlocks;
+    grid = cus * per_cu < kGramBlocks ? cus * per_cu : kGramBlocks;
+    cached.store(grid, std::memory_order_relaxed);
+  }
+  return grid;
+}
+
+template <int MA, int NB>
+int gram_launch(const ColDesc *descs, int kl, int kr, int share, int64_t d, double2 *partial,
+                hipStream_t s) {
+  const int grid = gram_grid<MA, NB>();
+  int64_t rpb = (d + grid - 1) / grid;
+  rpb = (rpb + 63) / 64 * 64;  // a whole number of tiles for every tile height
+  hipLaunchKernelGGL((k_gram_mfma<MA, NB>), dim3(grid), dim3(kGramThreads), 0, s, descs, kl, kr, share, d,
+                     rpb, partial);
+  return grid;
 }
 
 template <int MA>
-void gram_launch_nb(int nb, const ColDesc *descs, int kl, int kr, int share, int64_t d, int64_t rpb,
-                    double2 *partial, hipStream_t s) {
+int gram_launch_nb(int nb, const ColDesc *descs, int kl, int kr, int share, int64_t d, double2 *partial,
+                   hipStream_t s) {
   switch (nb) {
-    case 1: gram_launch<MA, 1>(descs, kl, kr, share, d, rpb, partial, s); break;
-    case 2: gram_launch<MA, 2>(descs, kl, kr, share, d, rpb, partial, s); break;
-    case 3: gram_launch<MA, 3>(descs, kl, kr, share, d, rpb, partial, s); break;
-    case 4: gram_launch<MA, 4>(descs, kl, kr, share, d, rpb, partial, s); break;
-    case 5: gram_launch<MA, 5>(descs, kl, kr, share, d, rpb, partial, s); break;
-    case 6: gram_launch<MA, 6>(descs, kl, kr, share, d, rpb, partial, s); break;
-    case 7: gram_launch<MA, 7>(descs, kl, kr, share, d, rpb, partial, s); break;
-    default: gram_launch<MA, 8>(descs, kl, kr, share, d, rpb, partial, s); break;
+    case 1: return gram_launch<MA, 1>(descs, kl, kr, share, d, partial, s);
+    case 2: return gram_launch<MA, 2>(descs, kl, kr, share, d, partial, s);
+    case 3: return gram_launch<MA, 3>(descs, kl, kr, share, d, partial, s);
+    case 4: return gram_launch<MA, 4>(descs, kl, kr, share, d, partial, s);
+    case 5: return gram_launch<MA, 5>(descs, kl, kr, share, d, partial, s);
+    case 6: return gram_launch<MA, 6>(descs, kl, kr, share, d, partial, s);
+    case 7: return gram_launch<MA, 7>(descs, kl, kr, share, d, partial, s);
+    default: return gram_launch<MA, 8>(descs, kl, kr, share, d, partial, s);
   }
 }
 }  // namespace
@@ -1170,19 +1198,18 @@ hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t 
                        Dbl2 *out, hipStream_t s) {
   if (kl < 1 || kr < 1 || kl > kGramMaxL || kr > kGramMaxR) return hipErrorInvalidValue;
   if (share < 0 || share > kl || share > kr) return hipErrorInvalidValue;
-  int64_t rpb = (d + kGramBlocks - 1) / kGramBlocks;
-  rpb = (rpb + 63) / 64 * 64;  // a whole number of tiles for every tile height
   const int ma = (kl + 15) / 16, nb = (kr + 15) / 16;
   double2 *pp = reinterpret_cast<double2 *>(partial);
+  int nparts = 0;
   switch (ma) {
-    case 1: gram_launch_nb<1>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
-    case 2: gram_launch_nb<2>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
-    case 3: gram_launch_nb<3>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
-    default: gram_launch_nb<4>(nb, descs, kl, kr, share, d, rpb, pp, s); break;
+    case 1: nparts = gram_launch_nb<1>(nb, descs, kl, kr, share, d, pp, s); break;
+    case 2: nparts = gram_launch_nb<2>(nb, descs, kl, kr, share, d, pp, s); break;
+    case 3: nparts = gram_launch_nb<3>(nb, descs, kl, kr, share, d, pp, s); break;
+    default: nparts = gram_launch_nb<4>(nb, descs, kl, kr, share, d, pp, s); break;
   }
   const int nelem = kl * kr;
   hipLaunchKernelGGL(k_reduce_partials, dim3((nelem + 3) / 4), dim3(256), 0, s,
-                     reinterpret_cast<const double2 *>(partial), kGramBlocks, nelem,
+                     reinterpret_cast<const double2 *>(partial), nparts, nelem,
                      reinterpret_cast<double2 *>(out));
   return hipGetLastError();
 }

@@ -162,7 +162,8 @@ class HeliumGrid:
     def sync(self):
         check(self._lib.dex_sync(self._ctx))
 
-    def set_kernel_timing(self, enable: bool = True):
+    def set_kernel_timing(self, enable: bool | int = True):
+        """False/0 off, True/1 every launch, n > 1 a random ~1 in n launches."""
         check(self._lib.dex_set_kernel_timing(self._ctx, int(enable)))
 
     def kernel_timing(self):

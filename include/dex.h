@@ -246,14 +246,18 @@ const char *dex_status_string(int status);
 const char *dex_last_error(void);
 
 /* Device event timing of the dominant matvec kernels of the most recent
- * dex_matvec_device call with timing enabled (ms per launch, -1 if none). */
+ * dex_matvec_device call with timing enabled (ms per launch, -1 if none).
+ * enable: 0 off, 1 every launch, n > 1 a random sample of about 1 in n
+ * launches for the kernel-class statistics below (each launch drawn
+ * independently; events cost about 10% of a solve's time when every launch is
+ * timed). */
 int dex_set_kernel_timing(dex_ctx *ctx, int32_t enable);
 int dex_get_kernel_timing(const dex_ctx *ctx, double *pass1_ms, double *pass2_ms);
 
-/* With timing enabled, every launch of the library's kernels is bracketed by
+/* With timing enabled, every timed launch of the library's kernels is bracketed by
  * HIP events on the context's stream (resolved lazily, so no synchronisation
  * is added to the work); per kernel class the summed device time, the number
- * of launches and the launches' algorithmic HBM bytes and flops (counted on
+ * of timed launches and their algorithmic HBM bytes and flops (counted on
  * the host from the launch shapes) accumulate until dex_reset_kernel_stats.  No reference
  * counterpart (the reference times with MPI_WTIME only, gplmr.F90:64-380). */
 enum {
