@@ -601,8 +601,11 @@ __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
 // workgroup share the output tiles (MT x NT of them) round-robin.
 typedef double dex_v4d __attribute__((ext_vector_type(4)));
 
+// Rows per staged tile: 16, halved while the tile exceeds 48 KB.  16-row tiles
+// (8-33 KB) leave room for 4+ resident workgroups per CU; 32-row tiles
+// measured 10-23% slower at 54-64 x 10 (2 workgroups per CU, VGPR-bound).
 constexpr int gram_mfma_rows(int mt, int nt) {
-  int r = 32;
+  int r = 16;
   while (r > 4 && r * (16 * mt + 16 * nt + 1) * 16 > 48 * 1024) r /= 2;
   return r;
 }
