@@ -60,6 +60,7 @@ struct dex_ctx {
   hipStream_t stream = nullptr;
   bool fast64 = false, fast32 = false;
   void *d_u = nullptr;  // matvec intermediate, D complex128 (also used as D complex64)
+  void *d_ub = nullptr;  // intermediates of a batched matvec launch: batch_cols(c) x D complex128
   ColDesc *d_desc = nullptr;
   unsigned char *d_args = nullptr;  // a combine's descriptors, C and beta, packed (one upload)
   Dbl2 *d_gram_out = nullptr;
@@ -177,6 +178,49 @@ hipError_t stage_upload(dex_ctx *c, void *dev, const void *src, size_t bytes) {
 int ensure_u(dex_ctx *c) {
   if (c->d_u) return 0;
   DEX_HIP(hipMalloc(&c->d_u, size_t(c->d) * sizeof(Dbl2)));
+  return 0;
+}
+
+// Columns per batched matvec launch: as many intermediates as fit in 512 MiB,
+// at most kBatchMax (16 at NN = 11, 6 at NN = 13, 2 at NN = 15, none from
+// NN = 17 on, where one column's passes already fill the device).
+int64_t batch_cols(const dex_ctx *c) {
+  const int64_t b = (int64_t(512) << 20) / (c->d * 16);
+  return b >= 2 ? std::min<int64_t>(b, dex::kBatchMax) : 1;
+}
+
+int ensure_ub(dex_ctx *c) {
+  if (c->d_ub || batch_cols(c) < 2) return 0;
+  DEX_HIP(hipMalloc(&c->d_ub, size_t(batch_cols(c)) * size_t(c->d) * sizeof(Dbl2)));
+  return 0;
+}
+
+// H on ncols complex128 columns with X = sym X^T (sym = +-1) in two launches
+// (batched electron-1 pass, batched transpose-add pass).
+int apply_h_batch(dex_ctx *c, const Dbl2 *const *x, Dbl2 *const *y, int ncols, bool single, int sym) {
+  if (ncols < 1 || ncols > batch_cols(c) || sym == 0 || !(single ? c->fast32 : c->fast64))
+    return fail(DEX_EINVAL, "batched matvec: bad column count or form");
+  if (int st = ensure_ub(c)) return st;
+  const int io = single ? dex::kIoMixed : dex::kIoF64;
+  const double dd = double(c->d) * ncols, nn = double(c->nn);
+  const double ub = single ? 8.0 : 16.0;
+  const void *xs[dex::kBatchMax];
+  void *us[dex::kBatchMax];
+  void *ys[dex::kBatchMax];
+  for (int j = 0; j < ncols; ++j) {
+    xs[j] = x[j];
+    us[j] = static_cast<char *>(c->d_ub) + size_t(j) * size_t(c->d) * sizeof(Dbl2);
+    ys[j] = y[j];
+  }
+  {
+    KTimer kt(c, DEX_K_PASS1, (16.0 + ub) * dd, 12.0 * nn * dd);
+    DEX_HIP(dex::launch_pass1_batch(c->nn, io, xs, us, ncols, c->stream));
+  }
+  {
+    KTimer kt(c, DEX_K_SYMADD, (32.0 + ub) * dd, 24.0 * dd);
+    DEX_HIP(dex::launch_symadd_batch(c->nn, io, xs, us, ys, ncols, c->d_vn, c->d_tinv, c->coef, sym,
+                                     c->stream));
+  }
   return 0;
 }
 
@@ -538,7 +582,7 @@ int dex_ctx_destroy(dex_ctx *c) {
   if (!c) return DEX_OK;
   (void)hipSetDevice(c->p.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (void *ptr : {(void *)c->d_tinv, (void *)c->d_vn, c->d_u, (void *)c->d_desc,
+  for (void *ptr : {(void *)c->d_tinv, (void *)c->d_vn, c->d_u, c->d_ub, (void *)c->d_desc,
                     (void *)c->d_args, (void *)c->d_gram_out, c->d_partial,
                     (void *)c->d_dots})
     if (ptr) (void)hipFree(ptr);
@@ -636,6 +680,23 @@ int dex_matvec_sym_device(dex_ctx *c, const void *in, void *out, int32_t howmany
   if (sp && c->p.precision != DEX_FP32) return fail(DEX_EINVAL, "complex64 I/O needs an FP32 context");
   if (sp && !c->fast32) return fail(DEX_EUNSUPPORTED, "no FP32 matvec for this grid");
   if (int st = ensure_u(c)) return st;
+  const int nb = int(batch_cols(c));
+  if (!sp && symmetry != 0 && !c->timing && howmany >= 2 && nb >= 2 &&
+      (c->p.precision == DEX_FP32 ? c->fast32 : c->fast64)) {
+    // exchange-symmetric complex128 columns: nb per pair of launches
+    std::vector<const Dbl2 *> xs(howmany);
+    std::vector<Dbl2 *> ys(howmany);
+    for (int j = 0; j < howmany; ++j) {
+      xs[j] = static_cast<const Dbl2 *>(in) + size_t(j) * c->d;
+      ys[j] = static_cast<Dbl2 *>(out) + size_t(j) * c->d;
+    }
+    for (int j0 = 0; j0 < howmany; j0 += nb) {
+      const int k = std::min(nb, howmany - j0);
+      if (int st = apply_h_batch(c, xs.data() + j0, ys.data() + j0, k, c->p.precision == DEX_FP32, symmetry))
+        return st;
+    }
+    return DEX_OK;
+  }
   for (int j = 0; j < howmany; ++j) {
     if (sp) {
       const char *xi = static_cast<const char *>(in) + size_t(j) * c->d * 8;
@@ -752,6 +813,18 @@ struct Solver {
 
   int matblock(const Block &in, const Block &out) {
     const double t0 = now_s();
+    const int nb = int(batch_cols(c));
+    if (in.fmt == 0 && sym != 0 && nb >= 2 && in.size() >= 2 && (single ? c->fast32 : c->fast64)) {
+      // exchange-symmetric columns: nb columns per pair of launches
+      for (int j0 = 0; j0 < in.size(); j0 += nb) {
+        const int k = std::min(nb, in.size() - j0);
+        if (int st = apply_h_batch(c, in.cols.data() + j0, out.cols.data() + j0, k, single, sym)) return st;
+        matvecs += k;
+      }
+      DEX_HIP(hipStreamSynchronize(c->stream));
+      t_matvec += now_s() - t0;
+      return 0;
+    }
     for (int j = 0; j < in.size(); ++j) {
       if (in.fmt != 0) {
         if (int st = apply_h32(c, in.cols[j], out.cols[j], sym)) return st;
@@ -1555,7 +1628,8 @@ extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns
                                        const dex_solve_opts *opts) {
   if (!c) return -1;
   // The caller's block X (NEV), the guard columns, the solver's (6 + 2 NS)
-  // (NEV + guard) work vectors, the matvec intermediate U, and for an inner
+  // (NEV + guard) work vectors, the matvec intermediate U (and the batched
+  // launches' intermediates, up to 512 MiB, on small grids), and for an inner
   // solve of order m on groups of g columns its Arnoldi basis ((m + 1) g) and
   // the next block (g).
   const int64_t vec = c->d * 16;
@@ -1564,7 +1638,9 @@ extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns
   const int64_t ib = opts ? opts->inner_block : 0;
   const int64_t grp = ib > 0 && ib < nblk ? ib : nblk;
   const int64_t inner = inner_iter >= 2 ? int64_t(inner_iter + 2) * grp : (inner_iter == 1 ? nblk : 0);
-  return (int64_t(7 + 2 * ns) * nblk + inner) * vec + vec;
+  const int64_t bc = batch_cols(c);
+  const int64_t batched = bc >= 2 ? bc * vec : 0;  // intermediates of the batched matvec launches
+  return (int64_t(7 + 2 * ns) * nblk + inner) * vec + vec + batched;
 }
 
 extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,

@@ -66,8 +66,19 @@ bool generic_matvec_supported(int nn);
 // intermediate U is complex64 in both FP32 forms).
 enum { kIoF64 = 0, kIoF32 = 1, kIoMixed = 2 };
 
+// Up to kBatchMax columns of one matvec launch, by value in the kernel arguments.
+constexpr int kBatchMax = 16;
+struct ColPtrs {
+  const void *p[kBatchMax];
+};
+struct ColPtrsW {
+  void *p[kBatchMax];
+};
+
 // Fast path: U = K1 X (electron 1), then Y = s2 (U + K2 X) + c1 Vr X.
 hipError_t launch_pass1(int nn, int io, const void *x, void *u, hipStream_t s);
+// pass 1 on ncols (1..kBatchMax) columns in one launch: U_j = K1 X_j
+hipError_t launch_pass1_batch(int nn, int io, const void *const *x, void *const *u, int ncols, hipStream_t s);
 hipError_t launch_pass2(int nn, int io, const void *x, const void *u, void *y,
                         const double *vn, const double *tinv, const MatvecCoef &coef,
                         hipStream_t s);
@@ -112,6 +123,10 @@ hipError_t launch_ywx(const Dbl2 *y, const Dbl2 *x, Dbl2 *w, int64_t m, int k, i
 hipError_t launch_symadd(int nn, int io, const void *x, const void *u, void *y,
                          const double *vn, const double *tinv, const MatvecCoef &coef, int sign,
                          hipStream_t s);
+// the same on ncols (1..kBatchMax) columns in one launch
+hipError_t launch_symadd_batch(int nn, int io, const void *const *x, const void *const *u, void *const *y,
+                               int ncols, const double *vn, const double *tinv, const MatvecCoef &coef,
+                               int sign, hipStream_t s);
 hipError_t launch_to_single(const Dbl2 *x, void *xs, int64_t n, hipStream_t s);
 hipError_t launch_to_double(const void *xs, Dbl2 *x, int64_t n, hipStream_t s);
 

@@ -119,8 +119,7 @@ __device__ __forceinline__ void cube_apply(const R *__restrict__ a, R *__restric
 // persistent: while a workgroup contracts cube q in LDS, the loads of its next
 // cube (q + gridDim.x) are already in flight into registers.
 template <int NN, typename R, typename XT>
-__global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(const XT *__restrict__ x,
-                                                              typename Cplx<R>::type *__restrict__ u) {
+__global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(ColPtrs xs, ColPtrsW us, int ncols) {
   using C = typename Cplx<R>::type;
   constexpr int NT = cube_threads<NN>();
   constexpr int N3 = NN * NN * NN;
@@ -129,14 +128,23 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(const XT *__restri
   R *a = reinterpret_cast<R *>(smem);
   R *b = a + N3;
   const int tid = threadIdx.x;
+  // work item q: cube q % N3 of column q / N3
+  const int64_t total = int64_t(ncols) * N3;
+  auto xcube = [&](int64_t qq) {
+    const int cc = int(qq / N3);
+    return static_cast<const XT *>(xs.p[cc]) + (qq - int64_t(cc) * N3) * N3;
+  };
   XT pre[PER];
   int64_t q = blockIdx.x;
+  if (q < total) {
+    const XT *xq = xcube(q);
 #pragma unroll
-  for (int t = 0; t < PER; ++t) {
-    const int e = tid + t * NT;
-    if (e < N3) pre[t] = x[q * N3 + e];
+    for (int t = 0; t < PER; ++t) {
+      const int e = tid + t * NT;
+      if (e < N3) pre[t] = xq[e];
+    }
   }
-  for (; q < N3; q += gridDim.x) {
+  for (; q < total; q += gridDim.x) {
     R keep[PER];
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
@@ -148,11 +156,12 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(const XT *__restri
     }
     __syncthreads();
     const int64_t qn = q + gridDim.x;
-    if (qn < N3) {
+    if (qn < total) {
+      const XT *xq = xcube(qn);
 #pragma unroll
       for (int t = 0; t < PER; ++t) {
         const int e = tid + t * NT;
-        if (e < N3) pre[t] = x[qn * N3 + e];
+        if (e < N3) pre[t] = xq[e];
       }
     }
     cube_apply<NN, NT, R>(a, b, tid);
@@ -167,7 +176,8 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass1(const XT *__restri
     }
     __syncthreads();
     cube_apply<NN, NT, R>(a, b, tid);
-    C *uq = u + q * N3;
+    const int cq = int(q / N3);
+    C *uq = static_cast<C *>(us.p[cq]) + (q - int64_t(cq) * N3) * N3;
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int e = tid + t * NT;
@@ -266,15 +276,17 @@ __global__ __launch_bounds__(cube_threads<NN>()) void k_pass2(
 constexpr int kSymT = 32;
 
 template <typename R, typename XT, typename YT>
-__global__ __launch_bounds__(256) void k_symadd(const XT *__restrict__ x,
-                                                const typename Cplx<R>::type *__restrict__ u,
-                                                YT *__restrict__ y, int64_t m, int nn,
+__global__ __launch_bounds__(256) void k_symadd(ColPtrs xs, ColPtrs us, ColPtrsW ys, int64_t m, int nn,
                                                 const double *__restrict__ vn,
                                                 const double *__restrict__ tinv, MatvecCoef cf,
                                                 int sign) {
   using C = typename Cplx<R>::type;
   const int ta = blockIdx.x, tb = blockIdx.y;
   if (ta > tb) return;
+  // column blockIdx.z of the launch
+  const XT *__restrict__ x = static_cast<const XT *>(xs.p[blockIdx.z]);
+  const C *__restrict__ u = static_cast<const C *>(us.p[blockIdx.z]);
+  YT *__restrict__ y = static_cast<YT *>(ys.p[blockIdx.z]);
   __shared__ C su[kSymT][kSymT + 1], sx[kSymT][kSymT + 1];
   __shared__ YT sy[kSymT][kSymT + 1];
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -396,8 +408,7 @@ hipError_t matvec_generic_nn(const double2 *x, double2 *y, const double *vn, con
 }
 
 template <int NN, typename R, typename XT>
-hipError_t pass1_nn(const void *x, void *u, hipStream_t s) {
-  using C = typename Cplx<R>::type;
+hipError_t pass1_nn(const ColPtrs &xs, const ColPtrsW &us, int ncols, hipStream_t s) {
   constexpr int N3 = NN * NN * NN;
   const size_t lds = size_t(2) * N3 * sizeof(R);
   static bool attr = false;
@@ -408,8 +419,8 @@ hipError_t pass1_nn(const void *x, void *u, hipStream_t s) {
     attr = true;
   }
   // persistent grid: as many workgroups as fit on the device at once
-  static int grid = 0;
-  if (grid == 0) {
+  static int resident = 0;
+  if (resident == 0) {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -417,10 +428,11 @@ hipError_t pass1_nn(const void *x, void *u, hipStream_t s) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
                                                      cube_threads<NN>(), lds) != hipSuccess || per < 1)
       per = 1;
-    grid = cus * per < N3 ? cus * per : N3;
+    resident = cus * per;
   }
-  hipLaunchKernelGGL((k_pass1<NN, R, XT>), dim3(grid), dim3(cube_threads<NN>()), lds, s,
-                     static_cast<const XT *>(x), static_cast<C *>(u));
+  const int64_t total = int64_t(ncols) * N3;
+  const int grid = int(resident < total ? resident : total);
+  hipLaunchKernelGGL((k_pass1<NN, R, XT>), dim3(grid), dim3(cube_threads<NN>()), lds, s, xs, us, ncols);
   return hipGetLastError();
 }
 
@@ -478,16 +490,23 @@ bool generic_matvec_supported(int nn) {
   }
 }
 
-hipError_t launch_pass1(int nn, int io, const void *x, void *u, hipStream_t s) {
+hipError_t launch_pass1_batch(int nn, int io, const void *const *x, void *const *u, int ncols, hipStream_t s) {
+  if (ncols < 1 || ncols > kBatchMax) return hipErrorInvalidValue;
+  ColPtrs xs{};
+  ColPtrsW us{};
+  for (int j = 0; j < ncols; ++j) {
+    xs.p[j] = x[j];
+    us.p[j] = u[j];
+  }
   switch (nn) {
 #define CASE(N)                                                                             \
   case N:                                                                                   \
     if constexpr (fast_fits(N, sizeof(double))) {                                           \
-      if (io == kIoF64) return pass1_nn<N, double, double2>(x, u, s);                       \
+      if (io == kIoF64) return pass1_nn<N, double, double2>(xs, us, ncols, s);              \
     }                                                                                       \
     if constexpr (fast_fits(N, sizeof(float))) {                                            \
-      if (io == kIoF32) return pass1_nn<N, float, float2>(x, u, s);                         \
-      if (io == kIoMixed) return pass1_nn<N, float, double2>(x, u, s);                      \
+      if (io == kIoF32) return pass1_nn<N, float, float2>(xs, us, ncols, s);                \
+      if (io == kIoMixed) return pass1_nn<N, float, double2>(xs, us, ncols, s);             \
     }                                                                                       \
     return hipErrorInvalidValue;
     DEX_NN_LIST(CASE)
@@ -495,6 +514,10 @@ hipError_t launch_pass1(int nn, int io, const void *x, void *u, hipStream_t s) {
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_pass1(int nn, int io, const void *x, void *u, hipStream_t s) {
+  return launch_pass1_batch(nn, io, &x, &u, 1, s);
 }
 
 hipError_t launch_pass2(int nn, int io, const void *x, const void *u, void *y,
@@ -518,25 +541,35 @@ hipError_t launch_pass2(int nn, int io, const void *x, const void *u, void *y,
   }
 }
 
-hipError_t launch_symadd(int nn, int io, const void *x, const void *u, void *y,
-                         const double *vn, const double *tinv, const MatvecCoef &coef, int sign,
-                         hipStream_t s) {
+hipError_t launch_symadd_batch(int nn, int io, const void *const *x, const void *const *u, void *const *y,
+                               int ncols, const double *vn, const double *tinv, const MatvecCoef &coef,
+                               int sign, hipStream_t s) {
+  if (ncols < 1 || ncols > kBatchMax) return hipErrorInvalidValue;
+  ColPtrs xs{}, us{};
+  ColPtrsW ys{};
+  for (int j = 0; j < ncols; ++j) {
+    xs.p[j] = x[j];
+    us.p[j] = u[j];
+    ys.p[j] = y[j];
+  }
   const int64_t m = int64_t(nn) * nn * nn;
   const unsigned nt = unsigned((m + kSymT - 1) / kSymT);
-  const dim3 g(nt, nt), b(kSymT, 8);
+  const dim3 g(nt, nt, unsigned(ncols)), b(kSymT, 8);
   if (io == kIoF32) {
-    hipLaunchKernelGGL((k_symadd<float, float2, float2>), g, b, 0, s, static_cast<const float2 *>(x),
-                       static_cast<const float2 *>(u), static_cast<float2 *>(y), m, nn, vn, tinv, coef, sign);
+    hipLaunchKernelGGL((k_symadd<float, float2, float2>), g, b, 0, s, xs, us, ys, m, nn, vn, tinv, coef, sign);
   } else if (io == kIoMixed) {
-    hipLaunchKernelGGL((k_symadd<float, double2, double2>), g, b, 0, s, static_cast<const double2 *>(x),
-                       static_cast<const float2 *>(u), static_cast<double2 *>(y), m, nn, vn, tinv, coef,
-                       sign);
+    hipLaunchKernelGGL((k_symadd<float, double2, double2>), g, b, 0, s, xs, us, ys, m, nn, vn, tinv, coef, sign);
   } else {
-    hipLaunchKernelGGL((k_symadd<double, double2, double2>), g, b, 0, s, static_cast<const double2 *>(x),
-                       static_cast<const double2 *>(u), static_cast<double2 *>(y), m, nn, vn, tinv, coef,
+    hipLaunchKernelGGL((k_symadd<double, double2, double2>), g, b, 0, s, xs, us, ys, m, nn, vn, tinv, coef,
                        sign);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_symadd(int nn, int io, const void *x, const void *u, void *y,
+                         const double *vn, const double *tinv, const MatvecCoef &coef, int sign,
+                         hipStream_t s) {
+  return launch_symadd_batch(nn, io, &x, &u, &y, 1, vn, tinv, coef, sign, s);
 }
 
 hipError_t launch_matvec_generic(int nn, const Dbl2 *x, Dbl2 *y, const double *vn,

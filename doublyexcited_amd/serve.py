@@ -66,13 +66,17 @@ def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1, inner_iter: i
                     inner_block: int = 0, guard: int = 0) -> int:
     """Device bytes a solve holds (the library's own count): the block X, the
     guard columns, (6 + 2 NS) (NEV + guard) work vectors, the matvec
-    intermediate (the same for both precisions: the basis is complex128) and,
-    for an inner solve of order m >= 2 on groups of g columns, (m + 2) g more."""
+    intermediate (the same for both precisions: the basis is complex128), the
+    batched matvec launches' intermediates (as many columns as fit in 512 MiB,
+    at most 16, when that is 2 or more) and, for an inner solve of order
+    m >= 2 on groups of g columns, (m + 2) g more."""
     d = (2 * n + 1) ** 6
     nblk = nev + guard
     grp = inner_block if 0 < inner_block < nblk else nblk
     inner = (inner_iter + 2) * grp if inner_iter >= 2 else (nblk if inner_iter == 1 else 0)
-    return ((7 + 2 * ns) * nblk + inner) * 16 * d + 16 * d
+    bc = min(16, (512 << 20) // (16 * d))
+    batched = bc * 16 * d if bc >= 2 else 0
+    return ((7 + 2 * ns) * nblk + inner) * 16 * d + 16 * d + batched
 
 
 def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),

@@ -165,6 +165,24 @@ def test_exchange_symmetric_matvec_matches_general(dex, n, sym):
         assert torch.equal(ysm, sym * ysm.t())
 
 
+@pytest.mark.parametrize("n,howmany", [(2, 5), (5, 20)])
+def test_batched_symmetric_matvec_matches_single_column(dex, n, howmany):
+    """Several exchange-symmetric columns in one call go through the batched
+    launches (up to 16 columns per pass; 20 columns = 16 + 4): bitwise the
+    per-column results."""
+    print(f"case: batched sym matvec N={n} howmany={howmany}", flush=True)
+    with dex.HeliumGrid(n, 0.5, 0.3) as g:
+        x = torch.randn(howmany, g.m, g.m, dtype=torch.complex128, device="cuda")
+        x = (0.5 * (x + x.transpose(1, 2))).contiguous().view(-1)
+        yb = torch.empty_like(x)
+        g.matvec_device(x, yb, howmany, symmetry=1)
+        y1 = torch.empty_like(x)
+        for j in range(howmany):
+            g.matvec_device(x[j * g.d:(j + 1) * g.d], y1[j * g.d:(j + 1) * g.d], 1, symmetry=1)
+        g.sync()
+        assert torch.equal(yb, y1)
+
+
 def test_symmetric_matvec_fp32(dex):
     print("case: sym matvec fp32 N=10", flush=True)
     with dex.HeliumGrid(10, 0.4, 0.35, precision="fp32") as g:
