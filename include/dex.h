@@ -114,7 +114,8 @@ typedef struct dex_result {
   int64_t matvecs;        /* HAMMULT applications */
   double max_residual;
   double t_total;         /* TOT_TIME (s) */
-  double t_matvec;        /* TSPMV (s), gplmr.F90:71-74 */
+  double t_matvec;        /* TSPMV (s), gplmr.F90:71-74: host time issuing the matvecs (they run
+                             asynchronously; device time per kernel class: dex_get_kernel_stats) */
   double t_prec;          /* TPREC (s) */
   double t_gram;          /* projection + orthogonalisation Grams (s) */
   double t_update;        /* block recombinations (s) */
