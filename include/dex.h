@@ -97,7 +97,10 @@ int dex_matvec_device(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
  * coalesced transpose-add; the result is Y = H Xs with Xs = (X + symmetry X^T)/2,
  * exactly (anti)symmetric.  This is how the reference's start vectors enter
  * (HELIUM_ORBITAL with SIGN = +-1, hammod.F90:1059-1063); HAMMULT itself has
- * no such mode.  Needs the LDS kernels for the grid (dex_info.fast_path). */
+ * no such mode.  Needs the LDS kernels for the grid (dex_info.fast_path).
+ * With howmany >= 2 on complex128 columns (and kernel timing off) the columns
+ * go through the passes in batches of up to 16 per launch, where the grid is
+ * small enough for their intermediates to fit in 512 MiB (NN <= 15). */
 int dex_matvec_sym_device(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
                           int64_t checksize, int32_t single, int32_t symmetry);
 
