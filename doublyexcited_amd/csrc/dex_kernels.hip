@@ -411,15 +411,16 @@ template <int NN, typename R, typename XT>
 hipError_t pass1_nn(const ColPtrs &xs, const ColPtrsW &us, int ncols, hipStream_t s) {
   constexpr int N3 = NN * NN * NN;
   const size_t lds = size_t(2) * N3 * sizeof(R);
-  static bool attr = false;
-  if (!attr) {
+  static std::atomic<bool> attr{false};  // set once per kernel; launches may come from several threads
+  if (!attr.load(std::memory_order_acquire)) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
-    attr = true;
+    attr.store(true, std::memory_order_release);
   }
   // persistent grid: as many workgroups as fit on the device at once
-  static int resident = 0;
+  static std::atomic<int> resident_cached{0};
+  int resident = resident_cached.load(std::memory_order_relaxed);
   if (resident == 0) {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -429,6 +430,7 @@ hipError_t pass1_nn(const ColPtrs &xs, const ColPtrsW &us, int ncols, hipStream_
                                                      cube_threads<NN>(), lds) != hipSuccess || per < 1)
       per = 1;
     resident = cus * per;
+    resident_cached.store(resident, std::memory_order_relaxed);
   }
   const int64_t total = int64_t(ncols) * N3;
   const int grid = int(resident < total ? resident : total);
@@ -442,12 +444,12 @@ hipError_t pass2_nn(const void *x, const void *u, void *y, const double *vn, con
   using C = typename Cplx<R>::type;
   constexpr int N3 = NN * NN * NN;
   const size_t lds = size_t(2) * N3 * sizeof(R);
-  static bool attr = false;
-  if (!attr) {
+  static std::atomic<bool> attr{false};  // set once per kernel; launches may come from several threads
+  if (!attr.load(std::memory_order_acquire)) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass2<NN, R, XT, YT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
-    attr = true;
+    attr.store(true, std::memory_order_release);
   }
   hipLaunchKernelGGL((k_pass2<NN, R, XT, YT>), dim3(N3), dim3(cube_threads<NN>()), lds, s,
                      static_cast<const XT *>(x), static_cast<const C *>(u), static_cast<YT *>(y), vn,
@@ -1255,12 +1257,12 @@ template <int JPT>
 hipError_t combine_launch(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
                           int64_t d, hipStream_t s) {
   const size_t lds = (size_t(nin) * kCombGroups * JPT + size_t(kCombRows) * nin) * sizeof(double2);
-  static bool attr = false;
-  if (!attr) {
+  static std::atomic<bool> attr{false};  // set once per kernel; launches may come from several threads
+  if (!attr.load(std::memory_order_acquire)) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_combine<JPT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
-    attr = true;
+    attr.store(true, std::memory_order_release);
   }
   const int64_t ntile = (d + kCombRows - 1) / kCombRows;
   const unsigned grid = unsigned(ntile < 2048 ? ntile : 2048);
