@@ -120,7 +120,8 @@ typedef struct dex_result {
   double t_matvec;        /* TSPMV (s), gplmr.F90:71-74: host time issuing the matvecs (they run
                              asynchronously; device time per kernel class: dex_get_kernel_stats) */
   double t_prec;          /* TPREC (s) */
-  double t_gram;          /* projection + orthogonalisation Grams (s) */
+  double t_gram;          /* projection + orthogonalisation Grams (s; host wall time, which includes
+                             waiting for the matvecs queued before each Gram) */
   double t_update;        /* block recombinations (s) */
   double t_smalleig;      /* host projected pencil (s) */
   int32_t fallbacks;      /* orthonormalisations that left the plain cholQR path */
