@@ -316,3 +316,37 @@ def test_singlet_mode_matches_general_solve(dex):
     for j in range(nev):
         xm = xb[:, j].reshape(h.m, h.m)
         assert np.array_equal(xm, xm.T)
+
+
+def test_kernel_statistics_every_and_sampled(dex):
+    """Kernel statistics: with timing 1 every launch is counted, with timing 4
+    a random subset (about a quarter) whose algorithmic bytes per launch and
+    device time per launch stay those of the full count; the solve itself is
+    unchanged by timing (same eigenvalues, same iteration count)."""
+    print("case: kernel statistics", flush=True)
+    h = Hamiltonian(2, 0.5, 0.3)
+    x0 = start_block(h, 4, 102)
+    out = {}
+    with dex.HeliumGrid(2, 0.5, 0.3) as g:
+        for period in (0, 1, 4):
+            g.reset_kernel_stats()
+            g.set_kernel_timing(period)
+            x = x0.copy(order="F")
+            r = g.gplmr_eig(x, -0.78 - 0.002j, 4, ns=1, maxit=40, tol=1e-14, symmetry=1)
+            g.set_kernel_timing(False)
+            out[period] = (r, g.kernel_stats())
+    r0, s0 = out[0]
+    r1, s1 = out[1]
+    r4, s4 = out[4]
+    assert all(v[1] == 0 for v in s0.values())
+    assert r1.iterations == r4.iterations == r0.iterations
+    assert np.allclose(r1.e, r0.e, atol=1e-12) and np.allclose(r4.e, r0.e, atol=1e-12)
+    for cls in ("pass1", "symadd", "gram", "combine"):
+        n1, n4 = s1[cls][1], s4[cls][1]
+        assert n1 > 30, (cls, n1)
+        assert 0.05 * n1 < n4 < 0.6 * n1, (cls, n1, n4)
+        # bytes per timed launch: the same population, sampled
+        b1, b4 = s1[cls][2] / n1, s4[cls][2] / n4
+        assert 0.5 * b1 < b4 < 2.0 * b1, cls
+        assert s1[cls][0] > 0 and s4[cls][0] > 0
+
