@@ -1266,7 +1266,8 @@ struct Solver {
     }
     for (int cc = 0; cc < k; ++cc)
       for (int i = 0; i < kb; ++i) rhs[i + size_t(rows) * cc] = r0[i + size_t(kb) * cc];
-    const int lst = dex::zlstsq(rows, cols, hb.data(), rows, rhs.data(), rows, k);
+    // block Hessenberg with upper triangular subdiagonal blocks: kb subdiagonals
+    const int lst = dex::zlstsq(rows, cols, hb.data(), rows, rhs.data(), rows, k, kb);
     t_small += now_s() - ts;
     if (lst != 0) {
       for (Dbl2 *p : qall.cols) pool.push_back(p);

@@ -28,7 +28,7 @@ void zheev_jacobi(int n, const std::complex<double> *a, int lda, std::vector<dou
 int zgeev_right(int n, const std::complex<double> *a, int lda, std::vector<std::complex<double>> &lam,
                 std::vector<std::complex<double>> &vec);
 int zlstsq(int rows, int cols, std::complex<double> *a, int lda, std::complex<double> *b, int ldb,
-           int nrhs);
+           int nrhs, int lower = -1);
 int harmonic_pencil(int k, const std::complex<double> *kp, const std::complex<double> *mp, int ld,
                     int nev, std::vector<std::complex<double>> &coords,
                     std::vector<std::complex<double>> &mu);

@@ -388,42 +388,48 @@ static int select_eigen(int n, const cd *a, int lda, int p, Key key, std::vector
   return 0;
 }
 // Least squares min ||A Y - B||_F for a full-column-rank A (rows >= cols),
-// by Householder QR (Golub & Van Loan 5.2.1, 5.3.3).  A (rows x cols, lda)
+// by Householder QR (Golub & Van Loan 5.2.1, 5.3.3).  With lower >= 0, A has
+// at most `lower` nonzero subdiagonals (a block Hessenberg matrix with upper
+// triangular subdiagonal blocks of order k has lower = k) and every
+// reflector spans lower + 1 rows; lower < 0: A is dense.  A (rows x cols, lda)
 // and B (rows x nrhs, ldb) are overwritten; Y is returned in the leading cols
 // rows of B.  Returns 0, or j + 1 when column j of the triangular factor is
 // numerically zero.
-int zlstsq(int rows, int cols, cd *a, int lda, cd *b, int ldb, int nrhs) {
+int zlstsq(int rows, int cols, cd *a, int lda, cd *b, int ldb, int nrhs, int lower) {
   if (rows < cols) return -1;
   double amax = 0.0;
   for (int j = 0; j < cols; ++j)
     for (int i = 0; i < rows; ++i) amax = std::max(amax, std::abs(a[i + size_t(lda) * j]));
   std::vector<cd> v(rows);
   for (int j = 0; j < cols; ++j) {
+    // rows j .. iend-1 of column j may be nonzero; reflections keep that band
+    // (they combine rows j .. j + lower, which later columns already span)
+    const int iend = lower < 0 ? rows : std::min(rows, j + lower + 1);
     double nrm2 = 0.0;
-    for (int i = j; i < rows; ++i) nrm2 += std::norm(a[i + size_t(lda) * j]);
+    for (int i = j; i < iend; ++i) nrm2 += std::norm(a[i + size_t(lda) * j]);
     const double nrm = std::sqrt(nrm2);
     if (nrm == 0.0) return j + 1;
     const cd x0 = a[j + size_t(lda) * j];
     const double ax0 = std::abs(x0);
     const cd ph = ax0 > 0.0 ? x0 / ax0 : cd(1.0, 0.0);
     const cd alpha = -ph * nrm;  // reflect x onto alpha e_1, |alpha| = ||x||
-    for (int i = j; i < rows; ++i) v[i] = a[i + size_t(lda) * j];
+    for (int i = j; i < iend; ++i) v[i] = a[i + size_t(lda) * j];
     v[j] -= alpha;
     double vn2 = 0.0;
-    for (int i = j; i < rows; ++i) vn2 += std::norm(v[i]);
+    for (int i = j; i < iend; ++i) vn2 += std::norm(v[i]);
     if (vn2 > 0.0) {
       // apply I - 2 v v^H / (v^H v) to the trailing columns of A and to B
       auto reflect = [&](cd *col) {
         cd dot = 0.0;
-        for (int i = j; i < rows; ++i) dot += std::conj(v[i]) * col[i];
+        for (int i = j; i < iend; ++i) dot += std::conj(v[i]) * col[i];
         const cd f = 2.0 * dot / vn2;
-        for (int i = j; i < rows; ++i) col[i] -= f * v[i];
+        for (int i = j; i < iend; ++i) col[i] -= f * v[i];
       };
       for (int jj = j + 1; jj < cols; ++jj) reflect(a + size_t(lda) * jj);
       for (int r = 0; r < nrhs; ++r) reflect(b + size_t(ldb) * r);
     }
     a[j + size_t(lda) * j] = alpha;
-    for (int i = j + 1; i < rows; ++i) a[i + size_t(lda) * j] = 0.0;
+    for (int i = j + 1; i < iend; ++i) a[i + size_t(lda) * j] = 0.0;
   }
   for (int j = 0; j < cols; ++j)
     if (!(std::abs(a[j + size_t(lda) * j]) > 1e-14 * amax)) return j + 1;
