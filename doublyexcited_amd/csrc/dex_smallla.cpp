@@ -15,6 +15,16 @@ namespace dex {
 
 using cd = std::complex<double>;
 
+// Complex products written out (a b and conj(a) b) for the O(n^3) inner
+// loops: std::complex's operator* adds an inf/NaN recovery branch per
+// product, which the finite data here never needs.
+static inline cd mul(const cd &a, const cd &b) {
+  return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+static inline cd mulcj(const cd &a, const cd &b) {
+  return cd(a.real() * b.real() + a.imag() * b.imag(), a.real() * b.imag() - a.imag() * b.real());
+}
+
 // Upper Cholesky factor A = R^H R (ZPOTRF 'U' semantics, lamod.f90:167).
 // Returns 0 or the 1-based index of the first non-positive pivot.  Only the
 // upper triangle of `a` is referenced; the strict lower triangle of the
@@ -145,21 +155,21 @@ static int zschur(int n, const cd *a_in, int lda, std::vector<cd> &h, std::vecto
     const double beta = 2.0 / un2;
     for (int j = 0; j < n; ++j) {  // H <- (I - beta u u^H) H
       cd s = 0.0;
-      for (int i = k + 1; i < n; ++i) s += std::conj(u[i]) * H(i, j);
+      for (int i = k + 1; i < n; ++i) s += mulcj(u[i], H(i, j));
       s *= beta;
-      for (int i = k + 1; i < n; ++i) H(i, j) -= u[i] * s;
+      for (int i = k + 1; i < n; ++i) H(i, j) -= mul(u[i], s);
     }
     for (int i = 0; i < n; ++i) {  // H <- H (I - beta u u^H), Z <- Z (I - beta u u^H)
       cd s = 0.0, t = 0.0;
       for (int j = k + 1; j < n; ++j) {
-        s += H(i, j) * u[j];
-        t += Z(i, j) * u[j];
+        s += mul(H(i, j), u[j]);
+        t += mul(Z(i, j), u[j]);
       }
       s *= beta;
       t *= beta;
       for (int j = k + 1; j < n; ++j) {
-        H(i, j) -= s * std::conj(u[j]);
-        Z(i, j) -= t * std::conj(u[j]);
+        H(i, j) -= mul(s, std::conj(u[j]));
+        Z(i, j) -= mul(t, std::conj(u[j]));
       }
     }
     for (int i = k + 2; i < n; ++i) H(i, k) = 0.0;
@@ -249,8 +259,8 @@ static int zschur(int n, const cd *a_in, int lda, std::vector<cd> &h, std::vecto
         sn[kk] = sv;
         for (int j = kk; j < n; ++j) {
           const cd hk = H(kk, j), hk1 = H(kk + 1, j);
-          H(kk, j) = c * hk + sv * hk1;
-          H(kk + 1, j) = -std::conj(sv) * hk + c * hk1;
+          H(kk, j) = c * hk + mul(sv, hk1);
+          H(kk + 1, j) = c * hk1 - mulcj(sv, hk);
         }
       }
       for (int kk = l; kk < i; ++kk) {
@@ -259,13 +269,13 @@ static int zschur(int n, const cd *a_in, int lda, std::vector<cd> &h, std::vecto
         const int rmax = std::min(kk + 2, i);
         for (int r = 0; r <= rmax; ++r) {
           const cd hk = H(r, kk), hk1 = H(r, kk + 1);
-          H(r, kk) = c * hk + std::conj(sv) * hk1;
-          H(r, kk + 1) = -sv * hk + c * hk1;
+          H(r, kk) = c * hk + mulcj(sv, hk1);
+          H(r, kk + 1) = c * hk1 - mul(sv, hk);
         }
         for (int r = 0; r < n; ++r) {
           const cd zk = Z(r, kk), zk1 = Z(r, kk + 1);
-          Z(r, kk) = c * zk + std::conj(sv) * zk1;
-          Z(r, kk + 1) = -sv * zk + c * zk1;
+          Z(r, kk) = c * zk + mulcj(sv, zk1);
+          Z(r, kk + 1) = c * zk1 - mul(sv, zk);
         }
       }
       for (int kk = l; kk <= i; ++kk) H(kk, kk) += mu;
@@ -505,13 +515,13 @@ int harmonic_pencil(int k, const cd *kp, const cd *mp, int ld, int nev, std::vec
       for (int j = 0; j < k; ++j)
         for (int i = 0; i < k; ++i) {
           cd s = 0.0;
-          for (int l = 0; l <= j; ++l) s += mp[i + size_t(ld) * l] * rinv[l + size_t(k) * j];
+          for (int l = 0; l <= j; ++l) s += mul(mp[i + size_t(ld) * l], rinv[l + size_t(k) * j]);
           t[i + size_t(k) * j] = s;
         }
       for (int j = 0; j < k; ++j)
         for (int i = 0; i < k; ++i) {
           cd s = 0.0;
-          for (int l = 0; l <= i; ++l) s += std::conj(rinv[l + size_t(k) * i]) * t[l + size_t(k) * j];
+          for (int l = 0; l <= i; ++l) s += mulcj(rinv[l + size_t(k) * i], t[l + size_t(k) * j]);
           c[i + size_t(k) * j] = s;
         }
       auto key = [](const cd &nu) { return std::abs(nu) > 0 ? 1.0 / std::abs(nu) : HUGE_VAL; };
