@@ -54,10 +54,10 @@ TIMING_SAMPLE = 8
 WORKLOADS = {
     # the plain CPU-runnable configuration of the reference method
     "he-nn11-block8": dict(n=5, delta=0.5, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-8,
-                           seed=1001, ns=3, inner=8, inner_block=0, guard=2, maxit=800, concurrency=1,
+                           seed=1001, ns=4, inner=8, inner_block=0, guard=2, maxit=800, concurrency=1,
                            desc="He, NN=11 (D=1.77M), delta 0.5, theta 0.3, 8 eigenpairs nearest "
                                 "sigma -0.78-0.002i (block 8 + 2 guard columns), tol 1e-8, "
-                                "NS 3, inner solve order 8"),
+                                "NS 4, inner solve order 8"),
     # the drop-in FP64 GPU configuration (a solve takes many minutes)
     "he-nn21-block16": dict(n=10, delta=0.4, theta=0.35, nev=16, sigma=complex(-0.70, -0.001), tol=1e-9,
                             seed=1002, ns=1, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
