@@ -256,14 +256,15 @@ def run_scan(w: dict, args) -> int:
               guard=w["guard"], maxit=w["maxit"], seed=w["seed"])
     if args.warmup:
         theta_scan(pts[:1], warm=False, **kw)
+    def report(p):
+        log(f"  NN={2 * p.n + 1} theta={p.theta:.4f}: {p.seconds:.2f}s iters {p.iterations} "
+            f"converged {p.converged_pairs}/{w['nev']} warm {p.warm}")
+
     t0 = time.time()
-    res = theta_scan(pts, warm=True, **kw)
+    res = theta_scan(pts, warm=True, on_point=report, **kw)
     wall = time.time() - t0
     pairs = sum(p.converged_pairs for p in res)
     lat = [p.seconds for p in res]
-    for p in res:
-        log(f"  NN={2 * p.n + 1} theta={p.theta:.4f}: {p.seconds:.2f}s iters {p.iterations} "
-            f"converged {p.converged_pairs}/{w['nev']} warm {p.warm}")
     print(json.dumps({
         "metric": "resonance eigenpairs/sec + p99 solve latency; mixed vs FP64 GPU at 1e-7 au",
         "value": pairs / wall if wall > 0 else 0.0, "unit": "eigenpairs/s", "n_gpus": 1,

@@ -32,11 +32,12 @@ class ScanPoint:
 def theta_scan(points, sigma: complex, nev: int, tol: float, ns: int = 1, inner_iter: int = 0,
                guard: int = 0, maxit: int = 1000, warm: bool = True, seed: int = 1004,
                precision: str = "fp64", checkpoint_dir: str | os.PathLike | None = None,
-               device: int = 0) -> list[ScanPoint]:
+               device: int = 0, on_point=None) -> list[ScanPoint]:
     """Solves the singlet resonances nearest ``sigma`` at each (n, delta, theta)
     of ``points`` in order.  Cold points start from seeded complex-Gaussian
     singlet blocks (seed + index); warm points from the previous point's
-    returned vectors."""
+    returned vectors.  ``on_point(point)``, if given, is called with each
+    ScanPoint as soon as it is solved (progress reporting)."""
     import torch
 
     from .eigvecs import sixd_fix, write_eigenvectors
@@ -61,6 +62,8 @@ def theta_scan(points, sigma: complex, nev: int, tol: float, ns: int = 1, inner_
             res = g.residuals_device(x, r.e)
             dt = time.time() - t0
             out.append(ScanPoint(n, delta, theta, r.e, res, r.iterations, int(np.sum(res <= tol)), dt, use_warm))
+            if on_point is not None:
+                on_point(out[-1])
             if checkpoint_dir is not None:
                 write_eigenvectors(os.path.join(checkpoint_dir, f"eigvecs.S.N{n}.T{theta:.4f}"),
                                    x.cpu().numpy().reshape(nev, -1))

@@ -183,6 +183,23 @@ def test_batched_symmetric_matvec_matches_single_column(dex, n, howmany):
         assert torch.equal(yb, y1)
 
 
+def test_batched_symmetric_matvec_fp32_context(dex):
+    """The batched launches in an FP32 context (FP32 arithmetic, complex128
+    columns): bitwise the per-column results."""
+    print("case: batched sym matvec fp32 context N=5", flush=True)
+    with dex.HeliumGrid(5, 0.5, 0.3, precision="fp32") as g:
+        howmany = 6
+        x = torch.randn(howmany, g.m, g.m, dtype=torch.complex128, device="cuda")
+        x = (0.5 * (x + x.transpose(1, 2))).contiguous().view(-1)
+        yb = torch.empty_like(x)
+        g.matvec_device(x, yb, howmany, symmetry=1)
+        y1 = torch.empty_like(x)
+        for j in range(howmany):
+            g.matvec_device(x[j * g.d:(j + 1) * g.d], y1[j * g.d:(j + 1) * g.d], 1, symmetry=1)
+        g.sync()
+        assert torch.equal(yb, y1)
+
+
 def test_symmetric_matvec_fp32(dex):
     print("case: sym matvec fp32 N=10", flush=True)
     with dex.HeliumGrid(10, 0.4, 0.35, precision="fp32") as g:
