@@ -34,7 +34,8 @@ EXPORTS = [
     "dex_get_kematrix", "dex_get_tinv", "dex_matvec", "dex_matvec_device", "dex_matvec_sym_device",
     "dex_dots",
     "dex_dots_device", "dex_solve_opts_default", "dex_gplmr_eig", "dex_gplmr_eig_device",
-    "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint", "dex_precblock_device", "dex_diagsubspace_device",
+    "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint",
+    "dex_solve_footprint", "dex_precblock_device", "dex_diagsubspace_device",
     "dex_status_string", "dex_last_error", "dex_set_kernel_timing", "dex_get_kernel_timing",
     "dex_get_kernel_stats", "dex_reset_kernel_stats",
     "dex_host_kematrix", "dex_host_getinverse", "dex_host_solve_gevp",
@@ -110,6 +111,8 @@ def load() -> C.CDLL:
         "dex_sync": (C.c_int, [P]),
         "dex_residuals_device": (C.c_int, [P, P, dp, C.c_int32, dp]),
         "dex_gplmr_footprint": (C.c_int64, [P, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+        "dex_solve_footprint": (C.c_int64, [C.POINTER(DexParams), C.c_int32, C.c_int32, C.c_int32,
+                                            C.c_void_p]),
         "dex_diagsubspace_device": (C.c_int, [P, dp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                               C.c_int32, C.c_void_p, dp, dp]),
         "dex_precblock_device": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -195,15 +196,17 @@ int ensure_ub(dex_ctx *c) {
   return 0;
 }
 
-// H on ncols complex128 columns with X = sym X^T (sym = +-1) in two launches
-// (batched electron-1 pass, batched transpose-add pass).
-int apply_h_batch(dex_ctx *c, const Dbl2 *const *x, Dbl2 *const *y, int ncols, bool single, int sym) {
+// H on ncols columns with X = sym X^T (sym = +-1) in two launches (batched
+// electron-1 pass, batched transpose-add pass).  io: dex::kIoF64 (complex128,
+// FP64), kIoMixed (complex128, FP32 arithmetic) or kIoF32 (complex64 columns).
+int apply_h_batch(dex_ctx *c, const Dbl2 *const *x, Dbl2 *const *y, int ncols, int io, int sym) {
+  const bool single = io != dex::kIoF64;
   if (ncols < 1 || ncols > batch_cols(c) || sym == 0 || !(single ? c->fast32 : c->fast64))
     return fail(DEX_EINVAL, "batched matvec: bad column count or form");
   if (int st = ensure_ub(c)) return st;
-  const int io = single ? dex::kIoMixed : dex::kIoF64;
   const double dd = double(c->d) * ncols, nn = double(c->nn);
-  const double ub = single ? 8.0 : 16.0;
+  const double ub = single ? 8.0 : 16.0;     // intermediate U entry bytes
+  const double xb = io == dex::kIoF32 ? 8.0 : 16.0;  // X and Y entry bytes
   const void *xs[dex::kBatchMax];
   void *us[dex::kBatchMax];
   void *ys[dex::kBatchMax];
@@ -213,11 +216,11 @@ int apply_h_batch(dex_ctx *c, const Dbl2 *const *x, Dbl2 *const *y, int ncols, b
     ys[j] = y[j];
   }
   {
-    KTimer kt(c, DEX_K_PASS1, (16.0 + ub) * dd, 12.0 * nn * dd);
+    KTimer kt(c, DEX_K_PASS1, (xb + ub) * dd, 12.0 * nn * dd);
     DEX_HIP(dex::launch_pass1_batch(c->nn, io, xs, us, ncols, c->stream));
   }
   {
-    KTimer kt(c, DEX_K_SYMADD, (32.0 + ub) * dd, 24.0 * dd);
+    KTimer kt(c, DEX_K_SYMADD, (2.0 * xb + ub) * dd, 24.0 * dd);
     DEX_HIP(dex::launch_symadd_batch(c->nn, io, xs, us, ys, ncols, c->d_vn, c->d_tinv, c->coef, sym,
                                      c->stream));
   }
@@ -285,17 +288,29 @@ int apply_h32(dex_ctx *c, const Dbl2 *x, Dbl2 *y, int sym) {
 }
 
 // ---- block helpers on one context ------------------------------------------
-// Columns of a block (device vectors of D entries) and their storage:
-// fmt 0 complex128, 1 complex64 (a complex64 column uses half of its buffer).
+// The solver's work vectors are complex128 (D x 16 B) or complex64 (D x 8 B)
+// buffers.  A complex64 buffer's pointer carries a tag in bit 0 (device
+// allocations are 256-byte aligned, so the bit is otherwise zero): blocks can
+// then mix storage formats column by column, and every descriptor built from a
+// pointer (plain, shifted) takes the untagged address and the format from it.
+inline bool is32(const void *p) { return (reinterpret_cast<uintptr_t>(p) & uintptr_t(1)) != 0; }
+inline Dbl2 *untag(const Dbl2 *p) {
+  return reinterpret_cast<Dbl2 *>(reinterpret_cast<uintptr_t>(p) & ~uintptr_t(1));
+}
+inline Dbl2 *tag32(Dbl2 *p) { return reinterpret_cast<Dbl2 *>(reinterpret_cast<uintptr_t>(p) | uintptr_t(1)); }
+
+// Columns of a block (device vectors of D entries).  `fmt` is the storage of
+// the columns grow() adds: 0 complex128, 1 complex64 (tagged pointers).
 struct Block {
   std::vector<Dbl2 *> cols;
   int fmt = 0;
   int size() const { return int(cols.size()); }
 };
 
-ColDesc plain(const Dbl2 *a, int fmt = 0) { return ColDesc{a, nullptr, 0.0, 0.0, -1, fmt}; }
-ColDesc shifted(const Dbl2 *a, const Dbl2 *b, cd sig, int fmt = 0) {
-  return ColDesc{a, b, sig.real(), sig.imag(), -1, fmt};
+ColDesc plain(const Dbl2 *a) { return ColDesc{untag(a), nullptr, 0.0, 0.0, -1, is32(a) ? 1 : 0}; }
+// a - sig b; a and b share one storage format
+ColDesc shifted(const Dbl2 *a, const Dbl2 *b, cd sig) {
+  return ColDesc{untag(a), untag(b), sig.real(), sig.imag(), -1, is32(a) ? 1 : 0};
 }
 
 int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
@@ -304,15 +319,19 @@ int upload_descs(dex_ctx *c, const std::vector<ColDesc> &ds) {
   return 0;
 }
 
-// Distinct vectors the descriptors read (a, and b where shifted).
-double distinct_vectors(const std::vector<ColDesc> &ds, size_t n) {
-  std::vector<const void *> v;
+// Bytes per entry summed over the distinct vectors the descriptors read (a, and
+// b where shifted): 16 for a complex128 column, 8 for a complex64 one.
+double distinct_entry_bytes(const std::vector<ColDesc> &ds, size_t n) {
+  std::vector<std::pair<const void *, int>> v;
   for (size_t i = 0; i < n && i < ds.size(); ++i) {
-    if (ds[i].a) v.push_back(ds[i].a);
-    if (ds[i].b) v.push_back(ds[i].b);
+    if (ds[i].a) v.push_back({ds[i].a, ds[i].fmt});
+    if (ds[i].b) v.push_back({ds[i].b, ds[i].fmt});
   }
   std::sort(v.begin(), v.end());
-  return double(std::unique(v.begin(), v.end()) - v.begin());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  double b = 0.0;
+  for (const auto &pv : v) b += pv.second ? 8.0 : 16.0;
+  return b;
 }
 
 // Runs one Gram launch (kl <= 64, kr <= 128) and copies the kl x kr result
@@ -323,7 +342,7 @@ int gram_launch_copy(dex_ctx *c, const std::vector<ColDesc> &ds, int kl, int kr,
   if (int st = upload_descs(c, ds)) return st;
   {
     const double hskip = share > 0 ? 0.5 * double(share) * double(share - 1) : 0.0;
-    KTimer kt(c, DEX_K_GRAM, 16.0 * double(c->d) * distinct_vectors(ds, ds.size()),
+    KTimer kt(c, DEX_K_GRAM, double(c->d) * distinct_entry_bytes(ds, ds.size()),
               8.0 * double(c->d) * (double(kl) * kr - hskip));
     DEX_HIP(dex::launch_gram(c->d_desc, kl, kr, share, c->d, reinterpret_cast<Dbl2 *>(c->d_partial),
                              c->d_gram_out, c->stream));
@@ -392,16 +411,21 @@ int gram_harmonic(dex_ctx *c, const std::vector<Dbl2 *> &q, const std::vector<Db
   return gram_launch_copy(c, ds, nq, 2 * nq, nq, g, nq, 0, 0);
 }
 
-// out_j = sum_i in_i C(i, j) (+ beta_j out_j), row-local, in-place safe.
+// out_j = sum_i in_i C(i, j) (+ beta_j out_j), row-local, in-place safe; each
+// output column is stored in its own pointer's format.
 int combine(dex_ctx *c, const std::vector<ColDesc> &in, const std::vector<Dbl2 *> &out,
-            const std::vector<cd> &cm, const std::vector<cd> *beta = nullptr, int out_fmt = 0) {
+            const std::vector<cd> &cm, const std::vector<cd> *beta = nullptr) {
   const int nin = int(in.size()), nout = int(out.size());
   if (nout == 0) return 0;
   if (nin * nout > kMaxSmall || cm.size() < size_t(nin) * nout || cm.size() > size_t(kMaxSmall) ||
       (beta && (beta->size() < size_t(nout) || beta->size() > size_t(kMaxSmall))))
     return fail(DEX_EINVAL, "combine matrix size");
   std::vector<ColDesc> ds(in);
-  for (Dbl2 *o : out) ds.push_back(plain(o, out_fmt));
+  double out_bytes = 0.0;
+  for (Dbl2 *o : out) {
+    ds.push_back(plain(o));
+    out_bytes += is32(o) ? 8.0 : 16.0;
+  }
   if (int(ds.size()) > kMaxDesc) return fail(DEX_EINVAL, "too many block columns");
   // descriptors | C | beta in one image, one copy
   const size_t bd = (ds.size() * sizeof(ColDesc) + 255) / 256 * 256;
@@ -413,7 +437,7 @@ int combine(dex_ctx *c, const std::vector<ColDesc> &in, const std::vector<Dbl2 *
   if (beta) std::memcpy(img.data() + bd + bc, beta->data(), bb);
   DEX_HIP(stage_upload(c, c->d_args, img.data(), img.size()));
   KTimer kt(c, DEX_K_COMBINE,
-            16.0 * double(c->d) * (distinct_vectors(in, in.size()) + double(nout) * (beta ? 2.0 : 1.0)),
+            double(c->d) * (distinct_entry_bytes(in, in.size()) + out_bytes * (beta ? 2.0 : 1.0)),
             8.0 * double(c->d) * nin * nout);
   const Dbl2 *dc = reinterpret_cast<const Dbl2 *>(c->d_args + bd);
   const Dbl2 *dbeta = beta ? reinterpret_cast<const Dbl2 *>(c->d_args + bd + bc) : nullptr;
@@ -434,7 +458,7 @@ int coldots(dex_ctx *c, const std::vector<ColDesc> &a, const std::vector<ColDesc
   }
   if (int st = upload_descs(c, ds)) return st;
   {
-    KTimer kt(c, DEX_K_COLDOTS, 16.0 * double(c->d) * distinct_vectors(ds, ds.size()),
+    KTimer kt(c, DEX_K_COLDOTS, double(c->d) * distinct_entry_bytes(ds, ds.size()),
               (b.empty() ? 4.0 : 16.0) * double(c->d) * k);
     DEX_HIP(dex::launch_coldots(c->d_desc, k, c->d, reinterpret_cast<double *>(c->d_partial), c->d_dots,
                                 c->stream));
@@ -455,7 +479,7 @@ int coldots(dex_ctx *c, const std::vector<ColDesc> &a, const std::vector<ColDesc
 
 std::vector<ColDesc> descs_of(const Block &b) {
   std::vector<ColDesc> v;
-  for (Dbl2 *p : b.cols) v.push_back(plain(p, b.fmt));
+  for (Dbl2 *p : b.cols) v.push_back(plain(p));
   return v;
 }
 
@@ -512,6 +536,12 @@ int dex_ctx_create(const dex_params *p, dex_ctx **out) {
   if (!dex::generic_matvec_supported(nn)) return fail(DEX_EUNSUPPORTED, "NN not built (odd 3..25)");
   if (p->nsmall * p->tfactor < nn) return fail(DEX_EINVAL, "NSMALL*TFACTOR < NN (tooth.f90:38)");
   DEX_HIP(hipSetDevice(p->device));
+  // kernel attributes and occupancy figures for every grid size, once per
+  // process and before any launch (launches may come from several threads)
+  static std::once_flag prepared;
+  static hipError_t prep_err = hipSuccess;
+  std::call_once(prepared, [] { prep_err = dex::prepare_kernels(); });
+  if (prep_err != hipSuccess) return hip_fail(prep_err, "kernel attribute set-up");
   auto *c = new dex_ctx();
   c->p = *p;
   c->nn = nn;
@@ -670,9 +700,10 @@ int dex_matvec_device(dex_ctx *c, const void *in, void *out, int32_t howmany, in
 int dex_matvec_sym_device(dex_ctx *c, const void *in, void *out, int32_t howmany, int64_t checksize,
                           int32_t single, int32_t symmetry) {
   if (symmetry < -1 || symmetry > 1) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
-  if (symmetry != 0 && !(c && (single ? c->fast32 : c->fast64)))
-    return fail(DEX_EUNSUPPORTED, "exchange-symmetric matvec needs the LDS kernels for this grid");
   if (!c || !in || !out) return fail(DEX_EINVAL, "null argument");
+  // an FP32 context applies the FP32 kernels to complex128 columns as well
+  if (symmetry != 0 && !((single || c->p.precision == DEX_FP32) ? c->fast32 : c->fast64))
+    return fail(DEX_EUNSUPPORTED, "exchange-symmetric matvec needs the LDS kernels for this grid");
   if (checksize != c->d) return fail(DEX_ESIZE, "SIZE ERROR: checksize != M*MBLK");
   if (howmany < 0) return fail(DEX_EINVAL, "howmany < 0");
   DEX_HIP(hipSetDevice(c->p.device));
@@ -692,7 +723,8 @@ int dex_matvec_sym_device(dex_ctx *c, const void *in, void *out, int32_t howmany
     }
     for (int j0 = 0; j0 < howmany; j0 += nb) {
       const int k = std::min(nb, howmany - j0);
-      if (int st = apply_h_batch(c, xs.data() + j0, ys.data() + j0, k, c->p.precision == DEX_FP32, symmetry))
+      if (int st = apply_h_batch(c, xs.data() + j0, ys.data() + j0, k,
+                                 c->p.precision == DEX_FP32 ? dex::kIoMixed : dex::kIoF64, symmetry))
         return st;
     }
     return DEX_OK;
@@ -803,31 +835,45 @@ struct Solver {
   dex_ctx *c;
   int nev, ns;
   bool strict;
-  bool single;  // FP32 matvec
   int sym;      // 0, or +-1 for exchange-(anti)symmetric blocks
   int fallbacks = 0;
   int64_t matvecs = 0;
   double t_matvec = 0, t_gram = 0, t_update = 0, t_small = 0, t_prec = 0;
   bool inner32 = false;  // inner solve: complex64 Arnoldi basis and FP32 matvecs
-  std::vector<Dbl2 *> pool;
+  std::vector<Dbl2 *> pool;    // free complex128 work vectors
+  std::vector<Dbl2 *> pool32;  // free complex64 work vectors (tagged pointers)
 
+  // H on every column of `in` into `out`.  complex128 columns get the FP64
+  // matvec, complex64 columns (both sides) the FP32 one.
   int matblock(const Block &in, const Block &out) {
     const double t0 = now_s();
+    if (out.size() < in.size()) return fail(DEX_EINVAL, "matblock: output block too small");
     const int nb = int(batch_cols(c));
-    if (in.fmt == 0 && sym != 0 && nb >= 2 && in.size() >= 2 && (single ? c->fast32 : c->fast64)) {
+    bool same = true;
+    for (int j = 0; j < in.size(); ++j)
+      if (is32(in.cols[j]) != is32(in.cols[0]) || is32(out.cols[j]) != is32(in.cols[j])) same = false;
+    const bool lo = in.size() > 0 && is32(in.cols[0]);
+    if (same && sym != 0 && nb >= 2 && in.size() >= 2 && (lo ? c->fast32 : c->fast64)) {
       // exchange-symmetric columns: nb columns per pair of launches
+      std::vector<Dbl2 *> xi, yo;
+      for (int j = 0; j < in.size(); ++j) {
+        xi.push_back(untag(in.cols[j]));
+        yo.push_back(untag(out.cols[j]));
+      }
       for (int j0 = 0; j0 < in.size(); j0 += nb) {
         const int k = std::min(nb, in.size() - j0);
-        if (int st = apply_h_batch(c, in.cols.data() + j0, out.cols.data() + j0, k, single, sym)) return st;
+        if (int st = apply_h_batch(c, xi.data() + j0, yo.data() + j0, k, lo ? dex::kIoF32 : dex::kIoF64, sym))
+          return st;
         matvecs += k;
       }
       t_matvec += now_s() - t0;
       return 0;
     }
     for (int j = 0; j < in.size(); ++j) {
-      if (in.fmt != 0) {
-        if (int st = apply_h32(c, in.cols[j], out.cols[j], sym)) return st;
-      } else if (int st = apply_h(c, in.cols[j], out.cols[j], single, sym)) {
+      if (is32(in.cols[j]) != is32(out.cols[j])) return fail(DEX_EINVAL, "matblock: mixed storage formats");
+      if (is32(in.cols[j])) {
+        if (int st = apply_h32(c, untag(in.cols[j]), untag(out.cols[j]), sym)) return st;
+      } else if (int st = apply_h(c, in.cols[j], out.cols[j], false, sym)) {
         return st;
       }
       ++matvecs;
@@ -919,18 +965,18 @@ struct Solver {
       if (kout > 0) {
         std::vector<ColDesc> in = descs_of(x);
         std::vector<Dbl2 *> out(x.cols.begin(), x.cols.begin() + kout);
-        if (int st = combine(c, in, out, t, nullptr, x.fmt)) return -st;
+        if (int st = combine(c, in, out, t, nullptr)) return -st;
         if (ax) {
           std::vector<ColDesc> ain = descs_of(*ax);
           std::vector<Dbl2 *> aout(ax->cols.begin(), ax->cols.begin() + kout);
-          if (int st = combine(c, ain, aout, t, nullptr, ax->fmt)) return -st;
+          if (int st = combine(c, ain, aout, t, nullptr)) return -st;
         }
       }
       t_update += now_s() - t0;
-      for (int j = kout; j < k; ++j) pool.push_back(x.cols[j]);
+      for (int j = kout; j < k; ++j) give(x.cols[j]);
       x.cols.resize(kout);
       if (ax) {
-        for (int j = kout; j < k; ++j) pool.push_back(ax->cols[j]);
+        for (int j = kout; j < k; ++j) give(ax->cols[j]);
         ax->cols.resize(kout);
       }
       if (fallbacks == before || kout == 0) break;  // plain cholQR path: one pass
@@ -962,13 +1008,13 @@ struct Solver {
       }
       std::vector<ColDesc> in;
       if (first) in = descs_of(b);
-      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i], q.fmt));
-      if (int st = combine(c, in, b.cols, cm, first ? nullptr : &one, b.fmt)) return st;
+      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i]));
+      if (int st = combine(c, in, b.cols, cm, first ? nullptr : &one)) return st;
       if (ab) {
         std::vector<ColDesc> ain;
         if (first) ain = descs_of(*ab);
-        for (int i = 0; i < nq; ++i) ain.push_back(plain(aq->cols[q0 + i], aq->fmt));
-        if (int st = combine(c, ain, ab->cols, cm, first ? nullptr : &one, ab->fmt)) return st;
+        for (int i = 0; i < nq; ++i) ain.push_back(plain(aq->cols[q0 + i]));
+        if (int st = combine(c, ain, ab->cols, cm, first ? nullptr : &one)) return st;
       }
       q0 += nq;
     }
@@ -985,12 +1031,12 @@ struct Solver {
     const double t0 = now_s();
     const int n1 = std::min(kq, 64 - k);
     std::vector<ColDesc> ds(w);
-    for (int i = 0; i < n1; ++i) ds.push_back(plain(q.cols[i], q.fmt));
+    for (int i = 0; i < n1; ++i) ds.push_back(plain(q.cols[i]));
     if (int st = gram_launch_copy(c, ds, k + n1, k, k, g, rows, 0, 0)) return st;
     for (int i0 = n1; i0 < kq; i0 += 64) {
       const int bl = std::min(64, kq - i0);
       std::vector<ColDesc> d2;
-      for (int i = 0; i < bl; ++i) d2.push_back(plain(q.cols[i0 + i], q.fmt));
+      for (int i = 0; i < bl; ++i) d2.push_back(plain(q.cols[i0 + i]));
       d2.insert(d2.end(), w.begin(), w.end());
       if (int st = gram_launch_copy(c, d2, bl, k, 0, g, rows, k + i0, 0)) return st;
     }
@@ -1028,8 +1074,8 @@ struct Solver {
       }
       std::vector<ColDesc> in;
       if (first) in = w;
-      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i], q.fmt));
-      if (int st = combine(c, in, out.cols, cm, first ? nullptr : &one, out.fmt)) return st;
+      for (int i = 0; i < nq; ++i) in.push_back(plain(q.cols[q0 + i]));
+      if (int st = combine(c, in, out.cols, cm, first ? nullptr : &one)) return st;
       q0 += nq;
       if (nq == 0) break;
     }
@@ -1237,7 +1283,7 @@ struct Solver {
       if (int st = grow(hq, kb)) return st;
       if (int st = matblock(qj, hq)) return st;
       std::vector<ColDesc> wd;
-      for (int i = 0; i < kb; ++i) wd.push_back(shifted(hq.cols[i], qj.cols[i], sigma, bf));
+      for (int i = 0; i < kb; ++i) wd.push_back(shifted(hq.cols[i], qj.cols[i], sigma));
       std::vector<cd> cj, rj;
       const bool last = j == m - 1;
       // a preconditioner's basis: one pass unless the projection cancels
@@ -1248,7 +1294,7 @@ struct Solver {
       hr.push_back(rj);
       ++nblk;
       if (!ok || last) {
-        for (Dbl2 *p : hq.cols) pool.push_back(p);
+        for (Dbl2 *p : hq.cols) give(p);
         break;
       }
       qall.cols.insert(qall.cols.end(), hq.cols.begin(), hq.cols.end());
@@ -1270,7 +1316,7 @@ struct Solver {
     const int lst = dex::zlstsq(rows, cols, hb.data(), rows, rhs.data(), rows, k, kb);
     t_small += now_s() - ts;
     if (lst != 0) {
-      for (Dbl2 *p : qall.cols) pool.push_back(p);
+      for (Dbl2 *p : qall.cols) give(p);
       return fail(DEX_EGEVP, "inner solve: projected least-squares problem is singular");
     }
     std::vector<cd> y(size_t(cols) * k);
@@ -1282,7 +1328,7 @@ struct Solver {
     const double tu = now_s();
     if (int st = combine_chunked(qm, y, b)) return st;
     t_update += now_s() - tu;
-    for (Dbl2 *p : qall.cols) pool.push_back(p);
+    for (Dbl2 *p : qall.cols) give(p);
     t_prec += now_s() - t0;
     return 0;
   }
@@ -1294,35 +1340,41 @@ struct Solver {
     for (int i0 = 0; i0 < nin; i0 += 64) {
       const int bi = std::min(64, nin - i0);
       std::vector<ColDesc> ds;
-      for (int i = 0; i < bi; ++i) ds.push_back(plain(in.cols[i0 + i], in.fmt));
+      for (int i = 0; i < bi; ++i) ds.push_back(plain(in.cols[i0 + i]));
       std::vector<cd> sub(size_t(bi) * nout);
       for (int j = 0; j < nout; ++j)
         for (int i = 0; i < bi; ++i) sub[i + size_t(bi) * j] = cm[i0 + i + size_t(nin) * j];
       std::vector<cd> one(nout, 1.0);
-      if (int st = combine(c, ds, out.cols, sub, i0 > 0 ? &one : nullptr, out.fmt)) return st;
+      if (int st = combine(c, ds, out.cols, sub, i0 > 0 ? &one : nullptr)) return st;
     }
     return 0;
   }
 
   std::vector<Dbl2 *> *store = nullptr;  // every buffer ever handed out (freed by the caller)
 
-  // A free work vector: from the pool, else a fresh allocation.  nullptr on OOM.
-  Dbl2 *take() {
-    if (!pool.empty()) {
-      Dbl2 *p = pool.back();
-      pool.pop_back();
+  // A free work vector of format fmt (0 complex128, 1 complex64, tagged): from
+  // its pool, else a fresh allocation.  nullptr on OOM.
+  Dbl2 *take(int fmt) {
+    std::vector<Dbl2 *> &pl = fmt ? pool32 : pool;
+    if (!pl.empty()) {
+      Dbl2 *p = pl.back();
+      pl.pop_back();
       return p;
     }
     Dbl2 *p = nullptr;
-    if (hipMalloc(&p, size_t(c->d) * sizeof(Dbl2)) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, size_t(c->d) * (fmt ? 8 : 16)) != hipSuccess) return nullptr;
+    if (fmt) p = tag32(p);
     store->push_back(p);
     return p;
   }
 
-  // Appends k work vectors to b, or returns DEX_EOOM.
+  // Returns a work vector to the pool of its format.
+  void give(Dbl2 *p) { (is32(p) ? pool32 : pool).push_back(p); }
+
+  // Appends k work vectors of b's format to b, or returns DEX_EOOM.
   int grow(Block &b, int k) {
     for (int j = 0; j < k; ++j) {
-      Dbl2 *p = take();
+      Dbl2 *p = take(b.fmt);
       if (!p) return fail(DEX_EOOM, "device allocation for GPLMR work vectors failed");
       b.cols.push_back(p);
     }
@@ -1333,22 +1385,24 @@ struct Solver {
 // One GPLMR run on the block `xcols` (the caller's columns, then any guard
 // columns); convergence and the returned values concern the first `nconv`
 // columns, which the harmonic-Ritz selection keeps ordered nearest-first.
+// lowp: the search blocks W, S, P, their images and the inner solve's basis are
+// complex64 with FP32 matvecs (FP64 accumulation in every Gram and
+// combination); X and AX stay complex128 and AX = H X is applied in FP64.
 int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out, cd sigma, int ns,
               int maxit, double tol, int inner, int outmodulus, const dex_solve_opts &o, dex_result *res,
-              bool single, std::vector<Dbl2 *> &store) {
+              bool lowp, std::vector<Dbl2 *> &store) {
   const int nev = int(xcols.size());
-  Solver S{c, nev, ns, o.strict_cholesky != 0, single, o.symmetry};
-  S.inner32 = o.inner_fp32 != 0;
-  // every buffer in `store` is free at entry
-  const size_t vbytes = size_t(c->d) * sizeof(Dbl2);
-  // columns: X in xcols; the rest from `store` (pool of (5 + 2 ns) nev vectors)
+  Solver S{c, nev, ns, o.strict_cholesky != 0, o.symmetry};
+  S.inner32 = o.inner_fp32 != 0 || lowp;
+  const int lf = lowp ? 1 : 0;  // storage of the search blocks
+  // columns: X in xcols; the rest from `store` (every buffer in it is free at entry)
   Block X, AX, W, AW, P, AP;
+  W.fmt = AW.fmt = P.fmt = AP.fmt = lf;
   X.cols = xcols;
-  S.pool = store;
+  for (Dbl2 *p : store) S.give(p);
   S.store = &store;
   if (int st = S.grow(AX, nev)) return st;
   if (int st = S.grow(W, nev)) return st;
-  (void)vbytes;
   const double t_start = now_s();
   if (o.symmetry != 0) {
     // the exchange-symmetric matvec assumes X = sym X^T: project the start block
@@ -1370,9 +1424,10 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     if (inner == 0 || b.size() == 0) return 0;
     if (inner == 1) {
       Block hb;
+      hb.fmt = b.fmt;
       if (int st = S.grow(hb, b.size())) return st;
       if (int st = S.matblock(b, hb)) return st;
-      for (Dbl2 *p : b.cols) S.pool.push_back(p);
+      for (Dbl2 *p : b.cols) S.give(p);
       b = hb;
       return 0;
     }
@@ -1433,6 +1488,7 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
   double maxres = *std::max_element(resn.begin(), resn.begin() + nconv);
   int iter = 1;
   Block Sblk, ASblk;
+  Sblk.fmt = ASblk.fmt = lf;
   while (iter <= maxit && maxres > tol) {
     if (o.refresh_every > 0 && iter > 1 && (iter - 1) % o.refresh_every == 0) {
       if (int st = S.matblock(X, AX)) return st;
@@ -1443,11 +1499,12 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
       if (!o.lock || resn[j] > tol) active.push_back(j);
     {
       Block wa;
+      wa.fmt = lf;
       std::vector<bool> keep(nev, false);
       for (int j : active) keep[j] = true;
       for (int j = 0; j < nev; ++j) {
         if (keep[j]) wa.cols.push_back(W.cols[j]);
-        else S.pool.push_back(W.cols[j]);
+        else S.give(W.cols[j]);
       }
       W = wa;
     }
@@ -1468,13 +1525,14 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     if (int st = S.grow(AW, W.size())) return st;
     if (int st = S.matblock(W, AW)) return st;  // :108
     // S blocks (:112-172)
-    for (Dbl2 *p : Sblk.cols) S.pool.push_back(p);
-    for (Dbl2 *p : ASblk.cols) S.pool.push_back(p);
+    for (Dbl2 *p : Sblk.cols) S.give(p);
+    for (Dbl2 *p : ASblk.cols) S.give(p);
     Sblk.cols.clear();
     ASblk.cols.clear();
     Block prev = W, aprev = AW;
     for (int l = 1; l <= ns; ++l) {
       Block sl;
+      sl.fmt = lf;
       const int k = prev.size();
       if (int st = S.grow(sl, k)) return st;
       if (k > 0) {
@@ -1497,6 +1555,7 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
         if (int st = S.orth_against(xws, sl, nullptr, nullptr, true)) return st;
       }
       Block asl;
+      asl.fmt = lf;
       if (int st = S.grow(asl, sl.size())) return st;
       if (int st = S.matblock(sl, asl)) return st;
       Sblk.cols.insert(Sblk.cols.end(), sl.cols.begin(), sl.cols.end());
@@ -1542,6 +1601,7 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     const int nact = int(active.size());
     // inputs [X W S P], outputs [X, Pnew] with Pnew in the old P's storage.
     Block Pn, APn;
+    Pn.fmt = APn.fmt = lf;
     for (int j = 0; j < np && j < nact; ++j) {
       Pn.cols.push_back(P.cols[j]);
       APn.cols.push_back(AP.cols[j]);
@@ -1584,15 +1644,15 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
       }
     }
     for (int j = nact; j < np; ++j) {
-      S.pool.push_back(P.cols[j]);
-      S.pool.push_back(AP.cols[j]);
+      S.give(P.cols[j]);
+      S.give(AP.cols[j]);
     }
     P = Pn;
     AP = APn;
     DEX_HIP(hipStreamSynchronize(c->stream));
     S.t_update += now_s() - t0;
     // free W and AW columns (W is rebuilt as the residual in place of the old W storage)
-    for (Dbl2 *p : AW.cols) S.pool.push_back(p);
+    for (Dbl2 *p : AW.cols) S.give(p);
     AW.cols.clear();
     // W gets nev columns again
     if (int st = S.grow(W, nev - W.size())) return st;
@@ -1625,23 +1685,59 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
 
 }  // namespace
 
-extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns, int32_t inner_iter,
-                                       const dex_solve_opts *opts) {
-  if (!c) return -1;
-  // The caller's block X (NEV), the guard columns, the solver's (6 + 2 NS)
-  // (NEV + guard) work vectors, the matvec intermediate U (and the batched
-  // launches' intermediates, up to 512 MiB, on small grids), and for an inner
-  // solve of order m on groups of g columns its Arnoldi basis ((m + 1) g) and
-  // the next block (g).
-  const int64_t vec = c->d * 16;
+// Columns per batched matvec launch for vectors of d entries (batch_cols).
+int64_t batch_cols_d(int64_t d) {
+  const int64_t b = (int64_t(512) << 20) / (d * 16);
+  return b >= 2 ? std::min<int64_t>(b, dex::kBatchMax) : 1;
+}
+
+// Device bytes a context of grid nn holds apart from any solve: the tables,
+// the block kernels' descriptor, coefficient and partial-sum buffers (as
+// dex_ctx_create allocates them), and the matvec intermediates (U, and on small
+// grids those of the batched launches), which a solve's first matvec allocates.
+int64_t ctx_fixed_bytes(int nn) {
+  const int64_t m = int64_t(nn) * nn * nn, d = m * m, vec = d * 16;
+  const int64_t L = 2 * nn + 1;
+  const int64_t partial = int64_t(std::max(dex::gram_partial_elems(64, 128) * sizeof(Dbl2),
+                                           dex::coldots_partial_elems(64) * sizeof(double)));
+  const int64_t bc = batch_cols_d(d);
+  return (L * L * L + m) * 8 + int64_t(kMaxDesc) * int64_t(sizeof(ColDesc)) +
+         (int64_t(kMaxDesc) * int64_t(sizeof(ColDesc)) + 2 * kMaxSmall * 16 + 512) + 192 * 192 * 16 + partial +
+         64 * 4 * 8 + vec + (bc >= 2 ? bc * vec : 0);
+}
+
+// Bytes of a solve (nev, ns, inner_iter, opts) on a context of grid nn and the
+// given precision, the context's fixed buffers included.  The caller's block X
+// (NEV) and the guard columns, AX, the solver's (5 + 2 NS) (NEV + guard)
+// search-block vectors (W, S, P, their images and the new P), and for an inner
+// solve of order m on groups of g columns its Arnoldi basis ((m + 1) g) and the
+// next block (g).  X and AX are complex128; the search blocks and the inner
+// basis are complex64 in the mixed form (FP32 context), the inner basis also
+// with inner_fp32.
+int64_t solve_footprint(int nn, int precision, int nev, int ns, int inner_iter, const dex_solve_opts *opts) {
+  const int64_t d = int64_t(nn) * nn * nn * nn * nn * nn;
+  const int64_t vec = d * 16;
+  const bool mixed = precision == DEX_FP32;
+  const int64_t lo = mixed ? d * 8 : vec;
   const int64_t guard = opts && opts->guard > 0 ? opts->guard : 0;
   const int64_t nblk = nev + guard;
   const int64_t ib = opts ? opts->inner_block : 0;
   const int64_t grp = ib > 0 && ib < nblk ? ib : nblk;
-  const int64_t inner = inner_iter >= 2 ? int64_t(inner_iter + 2) * grp : (inner_iter == 1 ? nblk : 0);
-  const int64_t bc = batch_cols(c);
-  const int64_t batched = bc >= 2 ? bc * vec : 0;  // intermediates of the batched matvec launches
-  return (int64_t(7 + 2 * ns) * nblk + inner) * vec + vec + batched;
+  const int64_t ivec = (mixed || (opts && opts->inner_fp32 != 0)) ? d * 8 : vec;
+  const int64_t inner = inner_iter >= 2 ? int64_t(inner_iter + 2) * grp * ivec : (inner_iter == 1 ? nblk * lo : 0);
+  return 2 * nblk * vec + int64_t(5 + 2 * ns) * nblk * lo + inner + ctx_fixed_bytes(nn);
+}
+
+extern "C" int64_t dex_gplmr_footprint(const dex_ctx *c, int32_t nev, int32_t ns, int32_t inner_iter,
+                                       const dex_solve_opts *opts) {
+  if (!c) return -1;
+  return solve_footprint(c->nn, c->p.precision, nev, ns, inner_iter, opts);
+}
+
+extern "C" int64_t dex_solve_footprint(const dex_params *p, int32_t nev, int32_t ns, int32_t inner_iter,
+                                       const dex_solve_opts *opts) {
+  if (!p || p->n < 1 || nev < 1 || ns < 1 || inner_iter < 0) return -1;
+  return solve_footprint(2 * p->n + 1, p->precision, nev, ns, inner_iter, opts);
 }
 
 extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t m, int32_t mb,
@@ -1666,17 +1762,27 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   if (opts) o = *opts;
   if (res) std::memset(res, 0, sizeof(*res));
   const int nblk = nev + guard;
-  const int nwork = (5 + 2 * ns) * nblk + nblk;  // pool plus headroom for the new P block
+  const bool mixed = c->p.precision == DEX_FP32;
+  // the work pool plus headroom for the new P block: AX (complex128) and
+  // (5 + 2 NS) (NEV + guard) search-block vectors, complex64 in the mixed form
+  const int nwork = (5 + 2 * ns) * nblk + nblk;
   std::vector<Dbl2 *> store, guards;
+  auto free_all = [&]() {
+    for (Dbl2 *q : store) (void)hipFree(untag(q));
+    for (Dbl2 *q : guards) (void)hipFree(q);
+  };
   for (int j = 0; j < nwork + guard; ++j) {
+    const bool lo = mixed && j >= nblk && j < nwork;
     Dbl2 *p = nullptr;
-    hipError_t err = hipMalloc(&p, size_t(c->d) * sizeof(Dbl2));
+    hipError_t err = hipMalloc(&p, size_t(c->d) * (lo ? 8 : 16));
     if (err != hipSuccess) {
-      for (Dbl2 *q : store) (void)hipFree(q);
-      for (Dbl2 *q : guards) (void)hipFree(q);
+      free_all();
       return fail(DEX_EOOM, "device allocation for GPLMR work vectors failed");
     }
-    (j < nwork ? store : guards).push_back(p);
+    if (j < nwork)
+      store.push_back(lo ? tag32(p) : p);
+    else
+      guards.push_back(p);
   }
   // the block: the caller's NEV columns, then the guard columns (seeded
   // complex Gaussians with the start block's exchange symmetry)
@@ -1690,9 +1796,12 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   const cd sigma(sigma_re, sigma_im);
   std::vector<cd> ev(nev);
   int st;
-  const bool mixed = c->p.precision == DEX_FP32;
-  // FP32 contexts: the FP32-matvec phase runs to max(tol, fp32_tol); FP64
-  // restarts from its block then reach tol.
+  // Mixed form (FP32 contexts): complex64 search blocks with FP32 matvecs
+  // around the complex128 X and AX.  AX is carried by the recurrence of the
+  // block updates, so the FP32 images leave it off H X by about the FP32
+  // rounding of the early steps; the first run therefore stops at
+  // max(tol, fp32_tol), and each restart recomputes AX = H X in FP64, so the
+  // residual it starts from (and tests) is the true one, and refines to tol.
   const double tol1 = mixed ? std::max(tol, o.fp32_tol) : tol;
   st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol1, inner_iter, outmodulus, o, res, mixed,
                  store);
@@ -1700,7 +1809,7 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
     for (int r = 0; r < o.fp64_restarts; ++r) {
       dex_result sub{};
       st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol, inner_iter, outmodulus, o,
-                     res ? &sub : nullptr, false, store);
+                     res ? &sub : nullptr, true, store);
       if (res) {
         res->matvecs += sub.matvecs;
         res->iterations += sub.iterations;
@@ -1715,11 +1824,12 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
         res->converged = sub.converged;
         res->fp64_restarts_used = r + 1;
       }
-      if (st != DEX_ENOTCONVERGED) break;
+      // a restart that starts converged (true residual <= tol) ends the refinement
+      if (st != DEX_ENOTCONVERGED && sub.iterations == 0) break;
+      if (st != DEX_OK && st != DEX_ENOTCONVERGED) break;
     }
   }
-  for (Dbl2 *q : store) (void)hipFree(q);
-  for (Dbl2 *q : guards) (void)hipFree(q);
+  free_all();
   for (int j = 0; j < nev; ++j) {
     e[2 * j] = ev[j].real();
     e[2 * j + 1] = ev[j].imag();

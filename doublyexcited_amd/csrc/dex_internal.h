@@ -57,6 +57,10 @@ struct MatvecCoef {
   double inv_d3;         // 1 / delta^3 (electron-electron term)
 };
 
+// Sets every kernel's function attributes and caches the occupancy figures the
+// launchers use (once per process, before the first launch; thread-safe).
+hipError_t prepare_kernels();
+
 // Returns false if no fast (LDS two-pass) kernel exists for (nn, single).
 bool fast_matvec_supported(int nn, bool single);
 bool generic_matvec_supported(int nn);

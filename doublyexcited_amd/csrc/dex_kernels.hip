@@ -16,6 +16,7 @@
 // real, so the real and imaginary parts are contracted in turn through two
 // LDS buffers of NN^3 reals each (148 KB at NN = 21 in FP64).
 #include <atomic>
+#include <mutex>
 
 #include <hip/hip_runtime.h>
 
@@ -407,31 +408,62 @@ hipError_t matvec_generic_nn(const double2 *x, double2 *y, const double *vn, con
   return hipGetLastError();
 }
 
+// Function attributes and occupancy figures are set once per kernel, under
+// g_setup_mu: prepare_kernels() does all of them when the first context is
+// created, before any launch, so no hipFuncSetAttribute call ever runs while
+// another thread launches (the launchers below only read the cached values,
+// taking the lock themselves only if prepare_kernels() was never called).
+std::mutex g_setup_mu;
+
+// Workgroups of k_pass1<NN, R, XT> resident on the device at once (its
+// dynamic LDS attribute set on first use).  0 on a HIP error.
+template <int NN, typename R, typename XT>
+int pass1_setup() {
+  static std::atomic<int> resident_cached{0};
+  int resident = resident_cached.load(std::memory_order_acquire);
+  if (resident != 0) return resident;
+  std::lock_guard<std::mutex> lk(g_setup_mu);
+  resident = resident_cached.load(std::memory_order_acquire);
+  if (resident != 0) return resident;
+  constexpr int N3 = NN * NN * NN;
+  const size_t lds = size_t(2) * N3 * sizeof(R);
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+    return 0;
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
+                                                   cube_threads<NN>(), lds) != hipSuccess || per < 1)
+    per = 1;
+  resident = cus * per;
+  resident_cached.store(resident, std::memory_order_release);
+  return resident;
+}
+
+template <int NN, typename R, typename XT, typename YT>
+bool pass2_setup() {
+  static std::atomic<bool> attr{false};
+  if (attr.load(std::memory_order_acquire)) return true;
+  std::lock_guard<std::mutex> lk(g_setup_mu);
+  if (attr.load(std::memory_order_acquire)) return true;
+  constexpr int N3 = NN * NN * NN;
+  const size_t lds = size_t(2) * N3 * sizeof(R);
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass2<NN, R, XT, YT>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+    return false;
+  attr.store(true, std::memory_order_release);
+  return true;
+}
+
 template <int NN, typename R, typename XT>
 hipError_t pass1_nn(const ColPtrs &xs, const ColPtrsW &us, int ncols, hipStream_t s) {
   constexpr int N3 = NN * NN * NN;
   const size_t lds = size_t(2) * N3 * sizeof(R);
-  static std::atomic<bool> attr{false};  // set once per kernel; launches may come from several threads
-  if (!attr.load(std::memory_order_acquire)) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    if (e != hipSuccess) return e;
-    attr.store(true, std::memory_order_release);
-  }
   // persistent grid: as many workgroups as fit on the device at once
-  static std::atomic<int> resident_cached{0};
-  int resident = resident_cached.load(std::memory_order_relaxed);
-  if (resident == 0) {
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_pass1<NN, R, XT>),
-                                                     cube_threads<NN>(), lds) != hipSuccess || per < 1)
-      per = 1;
-    resident = cus * per;
-    resident_cached.store(resident, std::memory_order_relaxed);
-  }
+  const int resident = pass1_setup<NN, R, XT>();
+  if (resident == 0) return hipErrorInvalidDeviceFunction;
   const int64_t total = int64_t(ncols) * N3;
   const int grid = int(resident < total ? resident : total);
   hipLaunchKernelGGL((k_pass1<NN, R, XT>), dim3(grid), dim3(cube_threads<NN>()), lds, s, xs, us, ncols);
@@ -444,13 +476,7 @@ hipError_t pass2_nn(const void *x, const void *u, void *y, const double *vn, con
   using C = typename Cplx<R>::type;
   constexpr int N3 = NN * NN * NN;
   const size_t lds = size_t(2) * N3 * sizeof(R);
-  static std::atomic<bool> attr{false};  // set once per kernel; launches may come from several threads
-  if (!attr.load(std::memory_order_acquire)) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pass2<NN, R, XT, YT>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    if (e != hipSuccess) return e;
-    attr.store(true, std::memory_order_release);
-  }
+  if (!pass2_setup<NN, R, XT, YT>()) return hipErrorInvalidDeviceFunction;
   hipLaunchKernelGGL((k_pass2<NN, R, XT, YT>), dim3(N3), dim3(cube_threads<NN>()), lds, s,
                      static_cast<const XT *>(x), static_cast<const C *>(u), static_cast<YT *>(y), vn,
                      tinv, cf);
@@ -1190,19 +1216,33 @@ namespace {
 template <int MA, int NB>
 int gram_grid() {
   static std::atomic<int> cached{0};
-  int grid = cached.load(std::memory_order_relaxed);
-  if (grid == 0) {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_gram_mfma<MA, NB>),
-                                                     kGramThreads, 0) != hipSuccess ||
-        cus < 1 || per_cu < 1)
-      return kGramBlocks;
-    grid = cus * per_cu < kGramBlocks ? cus * per_cu : kGramBlocks;
-    cached.store(grid, std::memory_order_relaxed);
-  }
+  int grid = cached.load(std::memory_order_acquire);
+  if (grid != 0) return grid;
+  std::lock_guard<std::mutex> lk(g_setup_mu);
+  grid = cached.load(std::memory_order_acquire);
+  if (grid != 0) return grid;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_gram_mfma<MA, NB>),
+                                                   kGramThreads, 0) != hipSuccess ||
+      cus < 1 || per_cu < 1)
+    return kGramBlocks;
+  grid = cus * per_cu < kGramBlocks ? cus * per_cu : kGramBlocks;
+  cached.store(grid, std::memory_order_release);
   return grid;
+}
+
+template <int MA>
+void gram_grids_nb() {
+  (void)gram_grid<MA, 1>();
+  (void)gram_grid<MA, 2>();
+  (void)gram_grid<MA, 3>();
+  (void)gram_grid<MA, 4>();
+  (void)gram_grid<MA, 5>();
+  (void)gram_grid<MA, 6>();
+  (void)gram_grid<MA, 7>();
+  (void)gram_grid<MA, 8>();
 }
 
 template <int MA, int NB>
@@ -1254,16 +1294,23 @@ hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t 
 
 namespace {
 template <int JPT>
+bool combine_setup() {
+  static std::atomic<bool> attr{false};
+  if (attr.load(std::memory_order_acquire)) return true;
+  std::lock_guard<std::mutex> lk(g_setup_mu);
+  if (attr.load(std::memory_order_acquire)) return true;
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_combine<JPT>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+    return false;
+  attr.store(true, std::memory_order_release);
+  return true;
+}
+
+template <int JPT>
 hipError_t combine_launch(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
                           int64_t d, hipStream_t s) {
   const size_t lds = (size_t(nin) * kCombGroups * JPT + size_t(kCombRows) * nin) * sizeof(double2);
-  static std::atomic<bool> attr{false};  // set once per kernel; launches may come from several threads
-  if (!attr.load(std::memory_order_acquire)) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_combine<JPT>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    attr.store(true, std::memory_order_release);
-  }
+  if (!combine_setup<JPT>()) return hipErrorInvalidDeviceFunction;
   const int64_t ntile = (d + kCombRows - 1) / kCombRows;
   const unsigned grid = unsigned(ntile < 2048 ? ntile : 2048);
   hipLaunchKernelGGL((k_combine<JPT>), dim3(grid), dim3(kCombThreads), lds, s, descs, nin, nout,
@@ -1342,6 +1389,36 @@ hipError_t launch_to_double(const void *xs, Dbl2 *x, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_to_double, dim3(unsigned((n + bs - 1) / bs)), dim3(bs), 0, s,
                      static_cast<const float2 *>(xs), reinterpret_cast<double2 *>(x), n);
   return hipGetLastError();
+}
+
+namespace {
+template <int NN>
+bool prepare_nn() {
+  bool ok = true;
+  if constexpr (fast_fits(NN, sizeof(double))) {
+    ok = ok && pass1_setup<NN, double, double2>() != 0;
+    ok = ok && pass2_setup<NN, double, double2, double2>();
+  }
+  if constexpr (fast_fits(NN, sizeof(float))) {
+    ok = ok && pass1_setup<NN, float, float2>() != 0 && pass1_setup<NN, float, double2>() != 0;
+    ok = ok && pass2_setup<NN, float, float2, float2>() && pass2_setup<NN, float, double2, double2>();
+  }
+  return ok;
+}
+}  // namespace
+
+hipError_t prepare_kernels() {
+  bool ok = true;
+#define CASE(N) ok = prepare_nn<N>() && ok;
+  DEX_NN_LIST(CASE)
+#undef CASE
+  ok = combine_setup<3>() && ok;
+  ok = combine_setup<4>() && ok;
+  gram_grids_nb<1>();
+  gram_grids_nb<2>();
+  gram_grids_nb<3>();
+  gram_grids_nb<4>();
+  return ok ? hipSuccess : hipErrorInvalidDeviceFunction;
 }
 
 }  // namespace dex

@@ -42,6 +42,26 @@ def _dev(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _check_dev(t, ncols: int, d: int, device: int, single: bool = False, name: str = "x"):
+    """Raises before any device call unless ``t`` is a contiguous tensor on this
+    context's GPU that holds ``ncols`` vectors of ``d`` entries in the format the
+    call reads or writes (complex128, or complex64 with ``single``): the library
+    takes a bare pointer and trusts its extent."""
+    import torch
+
+    want = torch.complex64 if single else torch.complex128
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor")
+    if t.device.index != device:
+        raise ValueError(f"{name} is on cuda:{t.device.index}, the context on cuda:{device}")
+    if t.dtype != want:
+        raise TypeError(f"{name} must be {want}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if ncols < 0 or t.numel() < ncols * d:
+        raise DexError(_lib.DEX_ESIZE, f"{name} holds {t.numel()} entries, the call needs {ncols} x {d}")
+
+
 @dataclasses.dataclass
 class SolveResult:
     e: np.ndarray
@@ -145,17 +165,22 @@ class HeliumGrid:
                       symmetry: int = 0) -> None:
         """HAMMULT on device tensors; symmetry = +1/-1 uses the exchange shortcut."""
         howmany = x.numel() // self.d if howmany is None else howmany
+        _check_dev(x, howmany, self.d, self.device, single, "x")
+        _check_dev(y, howmany, self.d, self.device, single, "y")
         check(self._lib.dex_matvec_sym_device(self._ctx, _dev(x), _dev(y), howmany, self.d,
                                               int(single), int(symmetry)))
         self.sync()
 
     def dots_device(self, bras, kets, nn: int, mm: int) -> np.ndarray:
         out = np.zeros((nn, mm), dtype=np.complex128, order="F")
+        _check_dev(bras, nn, self.d, self.device, name="bras")
+        _check_dev(kets, mm, self.d, self.device, name="kets")
         check(self._lib.dex_dots_device(self._ctx, _dev(bras), _dev(kets), self.d, nn, mm,
                                         out.ctypes.data_as(_dp)))
         return out
 
     def fill_gaussian_device(self, x, ncols: int, seed: int, stream_id: int = 0, symmetry: int = 1):
+        _check_dev(x, ncols, self.d, self.device)
         check(self._lib.dex_fill_gaussian_device(self._ctx, _dev(x), ncols, seed, stream_id,
                                                  int(symmetry)))
 
@@ -188,6 +213,7 @@ class HeliumGrid:
     def residuals_device(self, x, e: np.ndarray) -> np.ndarray:
         nev = e.size
         ee = np.ascontiguousarray(e, dtype=np.complex128)
+        _check_dev(x, nev, self.d, self.device)
         out = np.zeros(nev)
         check(self._lib.dex_residuals_device(self._ctx, _dev(x), ee.view(np.float64).ctypes.data_as(_dp),
                                              nev, _ptr(out)))
@@ -197,6 +223,10 @@ class HeliumGrid:
         """PRECBLOCK (gplmr.F90:466-502) on device blocks: px = H x for
         inner_iter = 1, else the order-inner_iter block minimal-residual
         approximation to (H - sigma)^{-1} x (PRECBLOCK_BASIC)."""
+        _check_dev(x, nev, self.d, self.device, name="x")
+        _check_dev(px, nev, self.d, self.device, name="px")
+        if x.data_ptr() == px.data_ptr():
+            raise ValueError("x and px must not alias")
         check(self._lib.dex_precblock_device(self._ctx, _dev(x), _dev(px), self.m, self.m, nev,
                                              float(sigma.real), float(sigma.imag), inner_iter, symmetry))
 
@@ -223,8 +253,8 @@ class HeliumGrid:
         return psi, e0, ex
 
     def footprint(self, nev: int, ns: int = 1, inner_iter: int = 0, inner_block: int = 0,
-                  guard: int = 0) -> int:
-        o = self._opts(False, False, 0, 2, 1e-5, 0, False, inner_block, guard)
+                  guard: int = 0, inner_fp32: bool = False) -> int:
+        o = self._opts(False, False, 0, 2, 1e-5, 0, False, inner_block, guard, inner_fp32)
         return int(self._lib.dex_gplmr_footprint(self._ctx, nev, ns, inner_iter, C.cast(C.byref(o), C.c_void_p)))
 
     # -- eigensolver --------------------------------------------------------------------
@@ -283,6 +313,7 @@ class HeliumGrid:
                          symmetry: int = 0, lock: bool = False, inner_block: int = 0,
                          guard: int = 0, inner_fp32: bool = False) -> SolveResult:
         """GPLMR_EIG with the block resident on the device (torch complex128 tensor)."""
+        _check_dev(x, nev, self.d, self.device)
         e = np.zeros(nev, dtype=np.complex128)
         r = _lib.DexResult()
         o = self._opts(strict, verbose, refresh_every, fp64_restarts, fp32_tol, symmetry, lock, inner_block,

@@ -63,20 +63,28 @@ def snap_nn(nn: int) -> int:
 
 
 def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1, inner_iter: int = 0,
-                    inner_block: int = 0, guard: int = 0) -> int:
-    """Device bytes a solve holds (the library's own count): the block X, the
-    guard columns, (6 + 2 NS) (NEV + guard) work vectors, the matvec
-    intermediate (the same for both precisions: the basis is complex128), the
-    batched matvec launches' intermediates (as many columns as fit in 512 MiB,
-    at most 16, when that is 2 or more) and, for an inner solve of order
-    m >= 2 on groups of g columns, (m + 2) g more."""
-    d = (2 * n + 1) ** 6
-    nblk = nev + guard
-    grp = inner_block if 0 < inner_block < nblk else nblk
-    inner = (inner_iter + 2) * grp if inner_iter >= 2 else (nblk if inner_iter == 1 else 0)
-    bc = min(16, (512 << 20) // (16 * d))
-    batched = bc * 16 * d if bc >= 2 else 0
-    return ((7 + 2 * ns) * nblk + inner) * 16 * d + 16 * d + batched
+                    inner_block: int = 0, guard: int = 0, inner_fp32: bool = False) -> int:
+    """Device bytes a solve holds, the library's own count (dex_solve_footprint,
+    no device needed): the caller's block X and the guard columns, AX, the
+    (5 + 2 NS) (NEV + guard) search-block vectors (complex64 in the mixed form),
+    the inner solve's basis for an order m >= 2 on groups of g columns
+    ((m + 2) g vectors) and the context's fixed buffers (tables, block-kernel
+    scratch, matvec intermediates)."""
+    import ctypes as C
+
+    from . import _lib
+    from .hamiltonian import HeliumGrid
+
+    lib = _lib.load()
+    p = _lib.DexParams()
+    lib.dex_params_default(C.byref(p))
+    p.n = n
+    p.precision = {"fp64": _lib.DEX_FP64, "fp32": _lib.DEX_FP32, "mixed": _lib.DEX_FP32}[precision]
+    o = HeliumGrid._opts(False, False, 0, 2, 1e-5, 0, False, inner_block, guard, inner_fp32)
+    b = int(lib.dex_solve_footprint(C.byref(p), nev, ns, inner_iter, C.cast(C.byref(o), C.c_void_p)))
+    if b < 0:
+        raise ValueError("invalid solve parameters")
+    return b
 
 
 def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),
@@ -126,7 +134,7 @@ class Server:
                 res = g.residuals_device(x, r.e)
                 del x
             done = time.time() - t0
-            conv = int(np.sum(res <= 1.5 * req.tol))
+            conv = int(np.sum(res <= req.tol))
             return Result(req.rid, True, r.e, res, conv, r.iterations, done - req.arrival,
                           done - t_admit)
         except Exception as exc:  # isolate the failure to this request

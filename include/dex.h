@@ -52,7 +52,8 @@ typedef struct dex_params {
   int32_t charge;     /* CHARGE (default 2, input.F90:40) */
   int32_t nsmall;     /* NSMALL (default 22, input.F90:31) */
   int32_t tfactor;    /* TFACTOR (default 4, input.F90:33) */
-  int32_t precision;  /* DEX_FP64 or DEX_FP32 arithmetic of the kinetic contraction */
+  int32_t precision;  /* DEX_FP64, or DEX_FP32: the mixed form (FP32 kinetic contraction on
+                         complex64 vectors for the solver's search blocks; see dex_gplmr_eig) */
   int32_t device;     /* HIP device ordinal */
 } dex_params;
 
@@ -132,8 +133,10 @@ typedef struct dex_solve_opts {
   int32_t strict_cholesky; /* 1: fail with DEX_ECHOLESKY where the reference STOPs */
   int32_t verbose;         /* print "EIG j = ..." every OUTMODULUS iterations like gplmr.F90:343-352 */
   int32_t refresh_every;   /* >0: recompute AX = H X every this many iterations (0 = never, as the reference) */
-  int32_t fp64_restarts;   /* FP32 contexts: at most this many FP64 restarts after the FP32 phase */
-  double fp32_tol;         /* FP32 contexts: residual target of the FP32-matvec phase */
+  int32_t fp64_restarts;   /* FP32 contexts: at most this many restarts from an AX = H X recomputed
+                              in FP64 after the first run (a restart that starts below TOL ends them) */
+  double fp32_tol;         /* FP32 contexts: residual target of the first run, whose AX is carried by
+                              the recurrence over FP32 images (default 1e-5) */
   int32_t symmetry;        /* 0: general HAMMULT; +1 / -1: project the start block to
                               singlet / triplet and use dex_matvec_sym_device */
   int32_t lock;            /* 1: build W, S, P only for columns whose residual is above
@@ -170,7 +173,15 @@ int dex_gplmr_eig(dex_ctx *ctx, double *x, double *e, int32_t m, int32_t mb,
                   double sigma_re, double sigma_im, int32_t nev, int32_t ns,
                   int32_t maxit, double tol, int32_t inner_iter, int32_t outmodulus,
                   const dex_solve_opts *opts, dex_result *res);
-/* Same with X resident on the device (complex128, M*MB*NEV). */
+/* Same with X resident on the device (complex128, M*MB*NEV).
+ *
+ * In an FP32 context (the mixed form) X and AX stay complex128 and AX = H X is
+ * applied in FP64, while the search blocks W, S, P, their images and the inner
+ * solve's Arnoldi basis are stored in complex64 and multiplied by H in FP32
+ * (FP32 accumulation); every Gram, combination, Cholesky factor and the
+ * projected pencil are FP64.  The first run stops at max(TOL, fp32_tol); each of
+ * up to fp64_restarts restarts recomputes AX = H X in FP64 and iterates to TOL.
+ * The reference is FP64 throughout (definitions.INC:1-2). */
 int dex_gplmr_eig_device(dex_ctx *ctx, void *x_dev, double *e, int32_t m, int32_t mb,
                          double sigma_re, double sigma_im, int32_t nev, int32_t ns,
                          int32_t maxit, double tol, int32_t inner_iter, int32_t outmodulus,
@@ -214,9 +225,16 @@ int dex_diagsubspace_device(dex_ctx *ctx, const double *orbitals, int32_t numorb
                             void *psi_dev, double *e0, double *expects);
 
 /* Bytes of device memory a dex_gplmr_eig_device solve of (nev, ns, inner_iter,
- * opts: inner_block and guard; NULL for the defaults) holds on this context,
- * the caller's block X included. */
+ * opts: inner_block, guard and inner_fp32; NULL for the defaults) holds on this
+ * context: the caller's block X, the solver's work vectors (complex64 search
+ * blocks in an FP32 context) and the context's own fixed buffers (tables,
+ * block-kernel scratch, matvec intermediates).  No reference counterpart (the
+ * reference allocates its work arrays inside GPLMR_EIG, gplmr.F90:31-36). */
 int64_t dex_gplmr_footprint(const dex_ctx *ctx, int32_t nev, int32_t ns, int32_t inner_iter,
+                            const dex_solve_opts *opts);
+/* The same figure from the parameters alone (p->n and p->precision), before
+ * any context exists and without a device: a request server's admission test. */
+int64_t dex_solve_footprint(const dex_params *p, int32_t nev, int32_t ns, int32_t inner_iter,
                             const dex_solve_opts *opts);
 
 /* Eigenvector files in the reference's format, READWRITE_EIGENVECTORS

@@ -24,16 +24,33 @@ def load():
         _lib.oracle_hammult.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, dp, C.c_int]
         _lib.oracle_hammult.restype = C.c_int
         _lib.oracle_max_threads.restype = C.c_int
+        _lib.oracle_potential2.argtypes = [C.c_int, C.c_int, dp, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, dp]
+        _lib.oracle_potential2.restype = C.c_int
     return _lib
+
+
+def potential2(h: Hamiltonian) -> np.ndarray:
+    """POTENTIAL2 / SCALE through the C restatement: V(P,Q) flat with P fastest."""
+    load()
+    dp = C.POINTER(C.c_double)
+    tinv = np.ascontiguousarray(h.tinv.transpose(2, 1, 0))  # [k][j][i]: i fastest in memory
+    v = np.empty(h.d, dtype=np.complex128)
+    _lib.oracle_potential2(h.nn, h.nout, tinv.ctypes.data_as(dp), float(h.z), h.delta, h.scale.real,
+                           h.scale.imag, v.ctypes.data_as(dp))
+    return v
 
 
 class CHammult:
     """HAMMULT through the C restatement; holds T, V and the KEMULT2 scratch."""
 
-    def __init__(self, h: Hamiltonian, nthreads: int = 0):
+    def __init__(self, h: Hamiltonian, nthreads: int = 0, c_potential: bool = False):
         self.h = h
         self.t = np.asfortranarray(h.t, dtype=np.complex128)
-        self.v = np.ascontiguousarray(h.potential2().reshape(-1, order="F"), dtype=np.complex128)
+        if c_potential:  # large grids: the C restatement of POTENTIAL2 (no (M, M, 3) index array)
+            self.v = potential2(h)
+        else:
+            self.v = np.ascontiguousarray(h.potential2().reshape(-1, order="F"), dtype=np.complex128)
         self.work = np.empty(2 * h.d, dtype=np.complex128)
         self.nthreads = nthreads
         load()

@@ -79,6 +79,36 @@ int oracle_hammult(int nn, const double *t_ri, const double *v_ri, const double 
   return 0;
 }
 
+/* POTENTIAL2 (hammod.F90:502-582) with the 1/SCALE of initmod.F90:224, one
+ * nucleus of charge z at the origin grid point:
+ *   V(P,Q) = [A Tinv(r_Q) + A Tinv(r_P) + Tinv(r_Q - r_P) / delta^3] / SCALE,
+ * A = -z / delta^3 (hammod.F90:554-555,571-573), r = (i, j, k) - N with i
+ * fastest (hammod.F90:560-562).  tinv: (2 nout + 1)^3 reals indexed
+ * [i + nout] fastest (the reference's TINV(-NOUT:NOUT)^3, nout = NN here,
+ * initmod.F90:39); v: D complex, V(P,Q) at P + M Q.  scale = (cos, sin)(theta). */
+int oracle_potential2(int nn, int nout, const double *tinv, double z, double delta, double scale_re,
+                      double scale_im, double *v_ri) {
+  const int n = (nn - 1) / 2;
+  const int64_t m = (int64_t)nn * nn * nn;
+  const int64_t L = 2 * (int64_t)nout + 1;
+  const double a = -z / (delta * delta * delta);
+  const double inv_d3 = 1.0 / (delta * delta * delta);
+  const zc inv_scale = 1.0 / (scale_re + I * scale_im);
+  zc *v = (zc *)v_ri;
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < m; ++q) {
+    const int qi = (int)(q % nn) - n, qj = (int)((q / nn) % nn) - n, qk = (int)(q / ((int64_t)nn * nn)) - n;
+    const double tq = tinv[(qi + nout) + L * ((qj + nout) + L * (qk + nout))];
+    for (int64_t p = 0; p < m; ++p) {
+      const int pi = (int)(p % nn) - n, pj = (int)((p / nn) % nn) - n, pk = (int)(p / ((int64_t)nn * nn)) - n;
+      const double tp = tinv[(pi + nout) + L * ((pj + nout) + L * (pk + nout))];
+      const double tee = tinv[(qi - pi + nout) + L * ((qj - pj + nout) + L * (qk - pk + nout))];
+      v[p + m * q] = (a * tq + a * tp + tee * inv_d3) * inv_scale;
+    }
+  }
+  return 0;
+}
+
 int oracle_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -4,8 +4,8 @@ Oracles: a dense eigendecomposition at NN=3, the numpy GPLMR restatement
 (oracle/gplmr.py) from the same start block at NN=5, and the reference's
 logged orbital-subspace value and singlet resonance at NN=11, delta=2,
 theta=0.2 (CSCALE-WORK/Outs/Out.S.N5:97-147).  Tolerances: |dE| <= 1e-9 au for FP64 solves
-converged to residual 1e-10, <= 1e-7 au for the mixed (FP32 matvec + FP64
-restarts) form.
+converged to residual 1e-10, <= 1e-7 au for the mixed form (complex64 search
+blocks with FP32 matvecs, FP64 restarts).
 """
 import json
 
@@ -226,21 +226,55 @@ def test_fp32_inner_solve_same_pairs(dex):
 
 
 def test_mixed_precision_within_1e7(dex):
-    """FP32 matvec phase to 1e-5, then FP64 restarts to 1e-9: |dE| <= 1e-7 au
-    against the FP64 solve from the same start block."""
+    """Mixed form (complex64 search blocks with FP32 matvecs around complex128
+    X and AX, then FP64 restarts): |dE| <= 1e-7 au against the oracle's FP64
+    GPLMR from the same start block, every returned residual recomputed with the
+    oracle's HAMMULT <= tol, and the FP64 solve on the GPU alike."""
     print("case: mixed N=2", flush=True)
     h = Hamiltonian(2, 0.5, 0.3)
-    nev, sigma = 4, -0.78 - 0.002j
+    v = h.potential2()
+    nev, sigma, tol = 4, -0.78 - 0.002j, 1e-9
     x0 = start_block(h, nev, 5)
+    ref = gplmr_eig(lambda b: np.stack([h.hammult(b[:, j], v) for j in range(b.shape[1])], axis=1),
+                    x0, sigma, nev, ns=1, maxit=800, tol=tol)
+    assert ref.converged
     with dex.HeliumGrid(2, 0.5, 0.3) as g:
         x = x0.copy(order="F")
-        r64 = g.gplmr_eig(x, sigma, nev, ns=1, maxit=800, tol=1e-9)
+        r64 = g.gplmr_eig(x, sigma, nev, ns=1, maxit=800, tol=tol)
     with dex.HeliumGrid(2, 0.5, 0.3, precision="mixed") as g:
         x = x0.copy(order="F")
-        r32 = g.gplmr_eig(x, sigma, nev, ns=1, maxit=800, tol=1e-9, fp32_tol=1e-5)
+        r32 = g.gplmr_eig(x, sigma, nev, ns=1, maxit=800, tol=tol, fp32_tol=1e-5)
     assert r64.converged and r32.converged
     assert r32.fp64_restarts_used >= 1
-    assert match(r32.e, r64.e) <= 1e-7
+    assert match(r32.e, ref.e) <= 1e-7 and match(ref.e, r32.e) <= 1e-7
+    assert match(r64.e, ref.e) <= 1e-9
+    for j in range(nev):
+        xj = x[:, j]
+        assert np.linalg.norm(h.hammult(xj, v) - r32.e[j] * xj) <= tol * np.linalg.norm(xj) * 1.01
+
+
+def test_mixed_precision_singlet_inner_solve(dex):
+    """Mixed form in the singlet mode with the inner solve (batched FP32
+    matvecs on complex64 columns, complex64 Arnoldi basis) and guard columns:
+    the same pairs as the FP64 solve (|dE| <= 1e-7 au), residuals recomputed
+    with the oracle's HAMMULT <= tol."""
+    print("case: mixed singlet inner N=3", flush=True)
+    h = Hamiltonian(3, 0.5, 0.3)
+    v = h.potential2()
+    nev, sigma, tol = 4, -0.78 - 0.002j, 1e-8
+    x0 = start_block(h, nev, 17)
+    kw = dict(ns=2, maxit=400, tol=tol, inner_iter=4, symmetry=1, guard=2)
+    with dex.HeliumGrid(3, 0.5, 0.3) as g:
+        x = x0.copy(order="F")
+        r64 = g.gplmr_eig(x, sigma, nev, **kw)
+    with dex.HeliumGrid(3, 0.5, 0.3, precision="mixed") as g:
+        x = x0.copy(order="F")
+        r32 = g.gplmr_eig(x, sigma, nev, **kw)
+    assert r64.converged and r32.converged
+    assert match(r32.e, r64.e) <= 1e-7 and match(r64.e, r32.e) <= 1e-7
+    for j in range(nev):
+        xj = x[:, j]
+        assert np.linalg.norm(h.hammult(xj, v) - r32.e[j] * xj) <= tol * np.linalg.norm(xj) * 1.01
 
 
 def test_diagsubspace_matches_oracle(dex):

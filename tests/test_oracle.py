@@ -172,6 +172,26 @@ def test_c_oracle_matches_numpy_oracle():
             assert np.max(np.abs(y[:, j] - ref)) <= 1e-14 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("n,delta,theta", [(1, 0.6, 0.3), (2, 0.45, 0.35), (3, 0.5, 0.3)])
+def test_c_potential2_matches_numpy_potential2(n, delta, theta):
+    """The C restatement of POTENTIAL2 (used at NN=21/25, where the numpy one
+    needs an (M, M, 3) index array) equals the numpy one, which the dense
+    Kronecker H and the reference's logged subspace eigenvalues pin; the e-e
+    term is what a swapped index or sign would break (V(P,Q) != V(Q,P) only
+    through the nuclear terms' placement, so the check is entrywise)."""
+    from oracle.c_oracle import CHammult, potential2
+
+    h = Hamiltonian(n, delta, theta)
+    ref = h.potential2().reshape(-1, order="F")
+    v = potential2(h)
+    assert np.max(np.abs(v - ref)) <= 1e-15 * np.max(np.abs(ref))
+    # and HAMMULT through it
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(h.d) + 1j * rng.standard_normal(h.d)
+    y = CHammult(h, c_potential=True)(x)
+    assert np.max(np.abs(y - h.hammult(x))) <= 1e-14 * np.max(np.abs(y))
+
+
 @pytest.mark.parametrize("order,k", [(1, 1), (1, 3), (2, 2), (4, 2)])
 def test_inner_solve_is_krylov_least_squares(order, k):
     """basicblocklansolve = argmin over span{b, Hb, ..., H^(m-1) b} of

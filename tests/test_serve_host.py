@@ -9,12 +9,24 @@ def test_snap_nn_rounds_even_grids_up():
     assert snap_nn(11) == 5 and snap_nn(12) == 6 and snap_nn(21) == 10 and snap_nn(20) == 10
 
 
-def test_footprint_matches_formula():
+def test_footprint_counts_blocks_and_context_buffers():
     d = 21 ** 6
-    assert footprint_bytes(10, 16, "fp64") == (7 + 2) * 16 * 16 * d + 16 * d
-    assert footprint_bytes(10, 16, "mixed") == footprint_bytes(10, 16, "fp64")
-    # NN=21, block 16 fits the 240 GB cap alone; two of them do not
+    vec = 16 * d
+    # the context's own buffers: the matvec intermediate U, the tables and the
+    # block kernels' scratch (a 134 MB partial-sum buffer among them)
+    fixed = footprint_bytes(10, 1, "fp64") - (2 + 7) * vec
+    assert fixed - vec > 134e6
+    # X and AX complex128, (5 + 2 NS) NEV search-block vectors: complex128, or
+    # complex64 in the mixed form
+    assert footprint_bytes(10, 16, "fp64") == (2 + 7) * 16 * vec + fixed
+    assert footprint_bytes(10, 16, "mixed") == 2 * 16 * vec + 7 * 16 * (vec // 2) + fixed
+    assert footprint_bytes(10, 16, "fp64", ns=2) - footprint_bytes(10, 16, "fp64") == 2 * 16 * vec
+    # an inner solve of order 4 on groups of 4 adds (4 + 2) 4 vectors of its format
+    assert footprint_bytes(10, 16, "fp64", 1, 4, 4) - footprint_bytes(10, 16, "fp64") == 6 * 4 * vec
+    assert footprint_bytes(10, 16, "mixed", 1, 4, 4) - footprint_bytes(10, 16, "mixed") == 6 * 4 * (vec // 2)
+    # NN=21, block 16 fits the 240 GB cap alone; two of them do not, two mixed ones do
     assert footprint_bytes(10, 16, "fp64") < 240e9 < 2 * footprint_bytes(10, 16, "fp64")
+    assert 2 * footprint_bytes(10, 16, "mixed") > 240e9 > footprint_bytes(10, 16, "mixed") + footprint_bytes(10, 8, "mixed")
 
 
 def test_poisson_stream_is_seeded_and_in_range():
