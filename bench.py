@@ -1,31 +1,40 @@
 #!/usr/bin/env python3
-"""Benchmark: converged resonance eigenpairs/s and solve latency of the GPLMR /
-harmonic-Ritz solver on one MI355X, FP64 beside the mixed form.
+"""Benchmark: converged resonance eigenpairs/s and p50/p99 solve latency of the
+GPLMR / harmonic-Ritz solver on one MI355X, mixed precision beside FP64.
 
-FP64 form: every matvec and block operation in FP64 (the inner solve too).
-Mixed form: the outer matvec in FP32 (complex128 vectors, complex64
-intermediate) until the residual reaches 1e-5, then FP64 restarts to the
-tolerance (with --mixed-inner-fp32 also the inner solve's Arnoldi basis in
-complex64 with FP32 matvecs).  Orthogonalisation, the projected eigenproblem
-and the final residuals stay FP64 in both forms.
+Default workload ``he-served-stream``: the served request stream.  K requests
+(``--steps``) arrive as a seeded Poisson process; each draws NN uniformly from
+11..21 (even draws snapped up to the next odd grid, the reference has no
+even-grid nucleus), spacing U[0.35, 0.6] bohr, theta U[0.2, 0.5] rad, Re sigma
+from {-0.78, -0.69, -0.60} au with Im sigma -0.002 au, block 4, 8 or 16, and
+tolerance 1e-7.  The server admits requests by device footprint under a 240 GB
+cap as slots and memory free up; each admitted solve has its own context and
+stream, so solves overlap on the device.  A step is one served request; its
+latency runs from arrival to completion.  Both forms serve the same requests:
 
-A step is one complete resonance solve of the selected workload: a seeded
-complex-Gaussian singlet start block taken by GPLMR (with the in-house inner
-minimal-residual solve) to the residual tolerance.  The Hamiltonian tables and
-every start block are resident in HBM before the timed region; up to
-``--concurrency`` solves are in flight at once, each on its own context and
-HIP stream (a small-grid solve alone leaves the GPU mostly idle).  ``value`` is
-the whole job's converged eigenpairs (final FP64 residual recomputed from
-scratch <= tol) per second of the timed region.
+  fp64   every matvec and block operation in FP64 (complex128 vectors);
+  mixed  X and AX complex128 (AX = H X in FP64), the search blocks W, S, P,
+         their images and the inner solve's basis complex64 with FP32 matvecs,
+         every Gram / combination / Cholesky / projected pencil in FP64, then
+         FP64 restarts (AX recomputed) to the tolerance.
+
+``value`` is the mixed form's converged eigenpairs (residual recomputed in FP64
+from scratch <= tol) per second of its timed region; the FP64 form's figures
+and the ratio are beside it, with max |dE| between the forms' eigenvalues.
 
 Also measured in the same run:
   roofline      the dominant kernel class of the timed solves, timed live with
-                HIP events on the library streams; algorithmic bytes (or flops)
-                per launch are counted on the host at each launch;
-  cpu_baseline  the C oracle's HAMMULT (oracle/hammult.c, OpenMP on the host
-                cores) timed on a bounded sample and scaled to the GPU solves'
-                matvec counts (block linear algebra not counted: an upper bound
-                on the CPU's eigenpairs/s).
+                HIP events on the library streams (a random ~1 in 8 launches);
+                algorithmic bytes (or flops) per launch are counted on the host
+                at each launch; the matvec's fraction of its bound per grid;
+  cpu_baseline  the oracle's FP64 path on the host cores: the C restatement of
+                HAMMULT timed per grid and numpy's block products timed at a
+                representative shape, applied to the FP64 solves' own counts
+                (matvecs per grid, block-kernel bytes), i.e. timed components of
+                a bounded sample scaled to the stream.
+
+The other workloads (``--workload``) are the single-configuration solves
+(NN=11 block 8, NN=21 block 16, NN=25 block 8) and the theta scan.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                        [--concurrency C] [--no-mixed] [--no-cpu-baseline]
@@ -278,15 +287,237 @@ def run_scan(w: dict, args) -> int:
     return 0
 
 
+# the served request stream (the metric's configuration)
+SERVED = dict(seed=1005, rate=0.5, cap=240e9, slots=4, tol=1e-7, maxit=100,
+              desc="served request stream: Poisson arrivals (seed 1005); NN uniform over 11..21 "
+                   "(even snapped up), delta U[0.35, 0.6], theta U[0.2, 0.5], Re sigma from "
+                   "{-0.78, -0.69, -0.60}, Im sigma -0.002, block from {4, 8, 16}, tol 1e-7; "
+                   "admission by device footprint under a 240 GB cap, up to 4 solves in flight")
+
+
+def _matvec_fraction_by_grid(results) -> dict:
+    """Per grid: device time of the matvec passes per column (event-timed
+    launches) against the bound max(F/peak, B/BW) of one column, with
+    F = (24 NN + 14) NN^6 flops and B = 32 NN^6 bytes (complex128 columns, FP64
+    peak) or 16 NN^6 (complex64 columns, FP32 peak)."""
+    acc = {}
+    for r in results:
+        if not r.ok or not r.kernel_stats:
+            continue
+        a = acc.setdefault(r.nn, [0.0, 0.0, 0.0])
+        for k in ("pass1", "pass2", "symadd"):
+            ms, n, by, fl = r.kernel_stats[k]
+            a[0] += ms
+        ms1, n1, by1, fl1 = r.kernel_stats["pass1"]
+        a[1] += by1  # pass-1 bytes = (X + U entry bytes) x D x columns
+        a[2] += fl1  # 12 NN D per column
+    out = {}
+    for nn, (ms, by1, fl1) in sorted(acc.items()):
+        d = float(nn) ** 6
+        cols = fl1 / (12.0 * nn * d) if fl1 > 0 else 0.0
+        if cols <= 0 or ms <= 0:
+            continue
+        per_col_s = ms * 1e-3 / cols
+        entry = by1 / (cols * d)  # 32 (FP64) or 16 (complex64 in and out) bytes per entry
+        fp32 = entry < 24.0
+        f = (24.0 * nn + 14.0) * d
+        b = (16.0 if fp32 else 32.0) * d
+        peak = (157.3 if fp32 else FP64_PEAK_TFS) * 1e12
+        bound = max(f / peak, b / (HBM_PEAK_GBS * 1e9))
+        out[str(nn)] = {"us_per_column": per_col_s * 1e6, "bound_us": bound * 1e6, "frac": bound / per_col_s,
+                        "precision": "fp32" if fp32 else "fp64", "columns_timed": round(cols, 1)}
+    return out
+
+
+def _roofline_from_results(results, sample: int) -> dict:
+    rows = [{"stats": r.kernel_stats} for r in results if r.ok and r.kernel_stats]
+    return algorithmic_roofline(rows, sample)
+
+
+def served_cpu_baseline(res64, budget_s: float = 25.0) -> dict:
+    """The oracle's FP64 path on the host: the C HAMMULT timed at every grid of
+    the stream and numpy's block products (Gram X^H Y and update X C, the
+    operations of the block kernels) timed at NN=15 with 16 columns; each FP64
+    request's CPU time is its matvec count x the HAMMULT time of its grid plus
+    its block kernels' algorithmic bytes / the numpy block-product bandwidth."""
+    import numpy as np
+
+    from oracle.c_oracle import CHammult
+    from oracle.hamiltonian import Hamiltonian
+
+    t_start = time.time()
+    grids = sorted({r.nn for r in res64 if r.ok})
+    t_mv = {}
+    for nn in grids:
+        h = Hamiltonian((nn - 1) // 2, 0.5, 0.3)
+        c = CHammult(h, c_potential=True)
+        x = np.ones(h.d, dtype=np.complex128)
+        c(x)
+        n, t0 = 0, time.time()
+        while n < 3 and (n == 0 or time.time() - t0 < 2.0):
+            c(x)
+            n += 1
+        t_mv[nn] = (time.time() - t0) / n
+        del c, x
+    # block products at NN=15 (D = 11.4M), 16 columns: bytes moved per second
+    d, k = 15 ** 6, 16
+    rng = np.random.default_rng(0)
+    xb = (rng.standard_normal((d, k)) + 1j * rng.standard_normal((d, k))).astype(np.complex128)
+    yb = xb[:, ::-1].copy()
+    cm = rng.standard_normal((k, k)) + 0j
+    xb.conj().T @ yb
+    n, t0 = 0, time.time()
+    while n < 2 or (time.time() - t0 < 3.0 and n < 10):
+        xb.conj().T @ yb   # reads 2 k columns
+        xb @ cm            # reads k, writes k columns
+        n += 1
+    la_bw = n * 4 * k * d * 16 / (time.time() - t0)
+    del xb, yb
+    cpu_s, pairs = 0.0, 0
+    for r in res64:
+        if not r.ok:
+            continue
+        la_bytes = sum(r.kernel_totals[k][1] for k in ("gram", "combine", "coldots"))
+        cpu_s += r.matvecs * t_mv[r.nn] + la_bytes / la_bw
+        pairs += r.converged_pairs
+    threads = CHammult.max_threads()
+    try:
+        import subprocess
+
+        lscpu = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True).stdout
+        phys = len({ln for ln in lscpu.splitlines() if ln and not ln.startswith("#")})
+    except Exception:
+        phys = None
+    return {
+        "value": pairs / cpu_s if cpu_s > 0 else None,
+        "unit": "eigenpairs/s",
+        "cores": threads,
+        "physical_cores": phys,
+        "kind": "port",
+        "sample": (f"C-oracle HAMMULT timed once per grid of the stream "
+                   f"({', '.join(f'NN={nn}: {t * 1e3:.0f} ms' for nn, t in t_mv.items())}; OpenMP, "
+                   f"{threads} threads) and numpy block products at NN=15 x 16 columns "
+                   f"({la_bw / 1e9:.1f} GB/s); applied to the FP64 solves' matvec counts and block-kernel "
+                   f"bytes: {cpu_s:.0f} s of CPU time for {pairs} pairs "
+                   f"(the sample took {time.time() - t_start:.0f} s)"),
+        "cpu_seconds_for_stream": cpu_s,
+    }
+
+
+def run_served(args) -> int:
+    import numpy as np
+    import torch
+
+    from doublyexcited_amd.serve import Request, Server, poisson_stream
+
+    torch.cuda.set_device(0)
+    sv = SERVED
+    forms = ["fp64"] if args.no_mixed else ["fp64", "mixed"]
+    slots = args.concurrency or sv["slots"]
+    out = {}
+    for form in forms:
+        if args.warmup:
+            warm = [Request(arrival=0.0, rid=i, n=5, delta=0.5, theta=0.3, sigma=-0.78 - 0.002j, nev=4,
+                            tol=1e-7, precision=form, seed=777 + i, inner_iter=8, guard=2, ns=4, maxit=400)
+                    for i in range(args.warmup)]
+            Server(sv["cap"], slots).run(warm, realtime=False)
+        reqs = poisson_stream(sv["rate"], args.steps, sv["seed"], precision=form, plan=True, cap=sv["cap"],
+                              maxit=args.maxit or sv["maxit"], tol=sv["tol"])
+        log(f"timed: {len(reqs)} {form} requests, rate {sv['rate']}/s, {slots} slots")
+        srv = Server(sv["cap"], slots, timing=TIMING_SAMPLE)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        res = srv.run(reqs, realtime=True, on_result=lambda r, q: log(
+            f"  {form} request {r.rid}: NN={r.nn} block {r.nev} ns {q.ns} inner {q.inner_iter}/{q.inner_block} "
+            f"guard {q.guard}: {r.converged_pairs}/{r.nev} pairs, {r.iterations} iterations, "
+            f"latency {r.latency:.1f}s service {r.service:.1f}s" + (f" ERROR {r.error}" if not r.ok else "")))
+        torch.cuda.synchronize()
+        wall = time.time() - t0
+        out[form] = (reqs, res, wall)
+    res64 = out["fp64"][1]
+    wall64 = out["fp64"][2]
+    lat64 = [r.latency for r in res64 if r.ok]
+    pairs64 = sum(r.converged_pairs for r in res64)
+    summary = {
+        "fp64": {"value": pairs64 / wall64, "unit": "eigenpairs/s", "wall_s": wall64, "converged_pairs": pairs64,
+                 "requested_pairs": sum(r.nev for r in out["fp64"][0]),
+                 "latency_s": {"p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99))},
+                 "failed_requests": sum(not r.ok for r in res64),
+                 "iterations": [r.iterations for r in res64]},
+    }
+    head = "fp64"
+    if "mixed" in out:
+        resmx, wallmx = out["mixed"][1], out["mixed"][2]
+        latmx = [r.latency for r in resmx if r.ok]
+        pairsmx = sum(r.converged_pairs for r in resmx)
+        de = [max(np.min(np.abs(np.asarray(a.e) - z)) for z in b.e)
+              for a, b in zip(res64, resmx) if a.ok and b.ok]
+        summary["mixed"] = {
+            "value": pairsmx / wallmx, "unit": "eigenpairs/s", "wall_s": wallmx, "converged_pairs": pairsmx,
+            "latency_s": {"p50": float(np.percentile(latmx, 50)), "p99": float(np.percentile(latmx, 99))},
+            "failed_requests": sum(not r.ok for r in resmx),
+            "fp64_restarts_mean": float(np.mean([r.fp64_restarts for r in resmx if r.ok])),
+            "iterations": [r.iterations for r in resmx],
+            "max_abs_dE_vs_fp64": float(max(de)) if de else None,
+            "ratio_vs_fp64": (pairsmx / wallmx) / (pairs64 / wall64) if pairs64 > 0 else None,
+        }
+        head = "mixed"
+    reqs, res, wall = out[head]
+    lat = [r.latency for r in res if r.ok]
+    pairs = sum(r.converged_pairs for r in res)
+    result = {
+        "metric": "resonance eigenpairs/sec + p99 solve latency; mixed vs FP64 GPU at 1e-7 au",
+        "value": pairs / wall if wall > 0 else 0.0,
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": ("complex64 search blocks (FP32 matvec) + complex128 X/AX (FP64)" if head == "mixed"
+                  else "complex128"),
+        "data": "synthetic: seeded request parameters and Philox complex-Gaussian singlet start blocks, "
+                "He Hamiltonian built per request",
+        "config": {"workload": "he-served-stream", "desc": SERVED["desc"], "form": head,
+                   "rate_per_s": SERVED["rate"], "requests": args.steps,
+                   "iteration_budget_per_request": args.maxit or SERVED["maxit"], "parallelism": "1 GPU"},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": float(max(lat))} if lat else None,
+        "converged_pairs": pairs,
+        "forms": summary,
+        "requests": [{"rid": q.rid, "nn": 2 * q.n + 1, "nev": q.nev, "ns": q.ns, "inner": q.inner_iter,
+                      "inner_block": q.inner_block, "guard": q.guard} for q in reqs],
+    }
+    result["roofline"] = _roofline_from_results(res, TIMING_SAMPLE)
+    rf = result["roofline"]
+    if rf.get("kernel"):
+        rf["traffic"] = traffic_from_profile("served", rf["kernel"], rf["alg_bytes_per_launch"])
+        rf["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
+    result["matvec_roofline_by_grid"] = {"fp64": _matvec_fraction_by_grid(res64)}
+    if "mixed" in out:
+        result["matvec_roofline_by_grid"]["mixed"] = _matvec_fraction_by_grid(out["mixed"][1])
+    if not args.no_cpu_baseline:
+        try:
+            result["cpu_baseline"] = served_cpu_baseline(res64)
+        except Exception as exc:  # reported, not fatal
+            result["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    print(json.dumps(result))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default=os.environ.get("DEX_BENCH_WORKLOAD", "he-nn11-block8"),
-                    choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=os.environ.get("DEX_BENCH_WORKLOAD", "he-served-stream"),
+                    choices=sorted(WORKLOADS) + ["he-served-stream"])
     ap.add_argument("--concurrency", type=int, default=0, help="solves in flight (0: workload default)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--maxit", type=int, default=0,
+                    help="served stream: iteration budget per request (0: the workload's, 100)")
     ap.add_argument("--mixed-inner-fp32", action="store_true",
                     help="mixed form: complex64 Arnoldi basis in the inner solve as well")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -297,6 +528,8 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: independent solves "
               "would shard across GPUs with no communication, and this build runs on one GPU")
         return 2
+    if args.workload == "he-served-stream":
+        return run_served(args)
     import numpy as np
     import torch
 

@@ -37,7 +37,7 @@ EXPORTS = [
     "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint",
     "dex_solve_footprint", "dex_precblock_device", "dex_diagsubspace_device",
     "dex_status_string", "dex_last_error", "dex_set_kernel_timing", "dex_get_kernel_timing",
-    "dex_get_kernel_stats", "dex_reset_kernel_stats",
+    "dex_get_kernel_stats", "dex_get_kernel_totals", "dex_reset_kernel_stats",
     "dex_host_kematrix", "dex_host_getinverse", "dex_host_solve_gevp",
     "dex_write_eigvecs", "dex_read_eigvecs_info", "dex_read_eigvecs", "dex_host_sixd_fix",
 ]
@@ -122,6 +122,7 @@ def load() -> C.CDLL:
         "dex_set_kernel_timing": (C.c_int, [P, C.c_int32]),
         "dex_get_kernel_timing": (C.c_int, [P, dp, dp]),
         "dex_get_kernel_stats": (C.c_int, [P, C.c_int32, dp, C.POINTER(C.c_int64), dp, dp]),
+        "dex_get_kernel_totals": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int64), dp, dp]),
         "dex_reset_kernel_stats": (C.c_int, [P]),
         "dex_host_kematrix": (C.c_int, [C.c_int32, C.c_double, dp]),
         "dex_write_eigvecs": (C.c_int, [C.c_char_p, dp, C.c_int32, C.c_int32]),

@@ -84,6 +84,10 @@ struct dex_ctx {
   int64_t k_n[DEX_KCLASSES] = {};
   double k_bytes[DEX_KCLASSES] = {};  // algorithmic HBM bytes of the timed launches
   double k_flops[DEX_KCLASSES] = {};  // algorithmic flops of the timed launches
+  // every launch, timed or not (counted on the host from the launch shapes)
+  int64_t k_all_n[DEX_KCLASSES] = {};
+  double k_all_bytes[DEX_KCLASSES] = {};
+  double k_all_flops[DEX_KCLASSES] = {};
   // pinned staging ring for the small host-to-device uploads (descriptors,
   // coefficients): copies from pageable memory would wait for the stream
   char *h_stage = nullptr;
@@ -139,6 +143,9 @@ struct KTimer {
   int cls;
   hipEvent_t a = nullptr;
   KTimer(dex_ctx *c_, int cls_, double bytes, double flops) : c(c_), cls(cls_) {
+    c->k_all_n[cls] += 1;
+    c->k_all_bytes[cls] += bytes;
+    c->k_all_flops[cls] += flops;
     if (c->timing && sampled(c) && (a = take_event(c)) != nullptr) {
       (void)hipEventRecord(a, c->stream);
       c->k_bytes[cls] += bytes;
@@ -674,6 +681,16 @@ int dex_get_kernel_stats(dex_ctx *c, int32_t kclass, double *total_ms, int64_t *
   return DEX_OK;
 }
 
+int dex_get_kernel_totals(const dex_ctx *c, int32_t kclass, int64_t *launches, double *alg_bytes,
+                          double *alg_flops) {
+  if (!c || !launches || !alg_bytes || !alg_flops) return fail(DEX_EINVAL, "null argument");
+  if (kclass < 0 || kclass >= DEX_KCLASSES) return fail(DEX_EINVAL, "unknown kernel class");
+  *launches = c->k_all_n[kclass];
+  *alg_bytes = c->k_all_bytes[kclass];
+  *alg_flops = c->k_all_flops[kclass];
+  return DEX_OK;
+}
+
 int dex_reset_kernel_stats(dex_ctx *c) {
   if (!c) return fail(DEX_EINVAL, "null argument");
   DEX_HIP(hipSetDevice(c->p.device));
@@ -681,6 +698,8 @@ int dex_reset_kernel_stats(dex_ctx *c) {
   for (int i = 0; i < DEX_KCLASSES; ++i) {
     c->k_ms[i] = c->k_bytes[i] = c->k_flops[i] = 0.0;
     c->k_n[i] = 0;
+    c->k_all_bytes[i] = c->k_all_flops[i] = 0.0;
+    c->k_all_n[i] = 0;
   }
   return DEX_OK;
 }

@@ -207,6 +207,16 @@ class HeliumGrid:
             out[name] = (ms.value, n.value, by.value, fl.value)
         return out
 
+    def kernel_totals(self) -> dict:
+        """{class: (launches, algorithmic bytes, algorithmic flops)} over every
+        launch since the context was created or the stats were reset."""
+        out = {}
+        for i, name in enumerate(_lib.KERNEL_CLASSES):
+            n, by, fl = C.c_int64(), C.c_double(), C.c_double()
+            check(self._lib.dex_get_kernel_totals(self._ctx, i, C.byref(n), C.byref(by), C.byref(fl)))
+            out[name] = (n.value, by.value, fl.value)
+        return out
+
     def reset_kernel_stats(self) -> None:
         check(self._lib.dex_reset_kernel_stats(self._ctx))
 

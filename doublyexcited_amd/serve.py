@@ -38,6 +38,7 @@ class Request:
     inner_iter: int = dataclasses.field(compare=False, default=0)   # GM_MAX_INNER_ITER
     inner_block: int = dataclasses.field(compare=False, default=0)
     guard: int = dataclasses.field(compare=False, default=0)
+    ns: int = dataclasses.field(compare=False, default=1)            # GPLMR_NS
 
 
 @dataclasses.dataclass
@@ -51,6 +52,12 @@ class Result:
     latency: float           # completion - arrival (s)
     service: float           # completion - admission (s)
     error: str | None = None
+    matvecs: int = 0
+    fp64_restarts: int = 0
+    nn: int = 0
+    nev: int = 0
+    kernel_stats: dict | None = None    # sampled event timings per kernel class
+    kernel_totals: dict | None = None   # every launch: (count, algorithmic bytes, flops) per class
 
 
 def snap_nn(nn: int) -> int:
@@ -87,30 +94,78 @@ def footprint_bytes(n: int, nev: int, precision: str, ns: int = 1, inner_iter: i
     return b
 
 
+def plan_solve(n: int, nev: int, precision: str, cap: float = 240e9) -> dict:
+    """Solver settings for a request: the widest GPLMR projection whose device
+    footprint fits the cap.
+
+    Guard columns (2 for blocks of 8 or fewer) let a block whose last wanted
+    level is degenerate hold the whole level; the inner solve (order 8 up to
+    NN=13, 6 above) is the reference's PRECBLOCK_BASIC; NS is as large as one
+    launch of the block kernels takes (NS (NEV + guard) <= 48, (3 + NS)
+    (NEV + guard) <= 96, at most 4).  Where that does not fit the cap, NS
+    drops, then the inner solve works on 4-column groups, then the guard
+    columns go, then the inner order falls.  The complex64 search blocks of the
+    mixed form fit wider projections than FP64 at the same grid."""
+    nn = 2 * n + 1
+    guard = 2 if nev <= 8 else 0
+    inner = 8 if nn <= 13 else 6
+    iblk = 0 if nn <= 15 else 4
+
+    def ns_max(g):
+        nb = nev + g
+        return max(1, min(4, 48 // nb, 96 // nb - 3))
+
+    ns = ns_max(guard)
+    while footprint_bytes(n, nev, precision, ns, inner, iblk, guard) > cap:
+        if ns > 1:
+            ns -= 1
+        elif iblk == 0 or iblk > 4:
+            iblk = 4
+        elif guard:
+            guard = 0
+            ns = ns_max(0)
+        elif inner > 2:
+            inner -= 2
+        else:
+            break
+    return dict(ns=ns, inner_iter=inner, inner_block=iblk, guard=guard)
+
+
 def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),
                    delta_range=(0.35, 0.6), theta_range=(0.2, 0.5),
                    sigma_re=(-0.78, -0.69, -0.60), sigma_im=-0.002, nev_choices=(4, 8, 16),
-                   tol=1e-7, precision="fp64"):
-    """Seeded request stream with Poisson arrivals."""
+                   tol=1e-7, precision="fp64", plan: bool = False, cap: float = 240e9, maxit: int = 3000):
+    """Seeded request stream with Poisson arrivals (rate per second).  With
+    ``plan`` each request carries the solver settings of ``plan_solve`` for
+    its precision and the cap."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(count):
         t += rng.exponential(1.0 / rate)
         nn = int(rng.choice(list(nn_choices)))
-        out.append(Request(arrival=t, rid=i, n=snap_nn(nn), delta=float(rng.uniform(*delta_range)),
-                           theta=float(rng.uniform(*theta_range)),
-                           sigma=complex(float(rng.choice(sigma_re)), sigma_im),
-                           nev=int(rng.choice(nev_choices)), tol=tol, precision=precision,
-                           seed=seed * 100003 + i))
+        r = Request(arrival=t, rid=i, n=snap_nn(nn), delta=float(rng.uniform(*delta_range)),
+                    theta=float(rng.uniform(*theta_range)),
+                    sigma=complex(float(rng.choice(sigma_re)), sigma_im),
+                    nev=int(rng.choice(nev_choices)), tol=tol, precision=precision,
+                    seed=seed * 100003 + i, maxit=maxit)
+        if plan:
+            for k, v in plan_solve(r.n, r.nev, precision, cap).items():
+                setattr(r, k, v)
+        out.append(r)
     return out
 
 
 class Server:
-    def __init__(self, hbm_cap_bytes: float = 240e9, slots: int = 4, device: int = 0):
+    """``timing`` > 0 brackets a random ~1 in ``timing`` launches of each
+    solve's kernels with HIP events (1: every launch) and returns the per-class
+    statistics with the result."""
+
+    def __init__(self, hbm_cap_bytes: float = 240e9, slots: int = 4, device: int = 0, timing: int = 0):
         self.cap = float(hbm_cap_bytes)
         self.slots = slots
         self.device = device
+        self.timing = int(timing)
         self._lock = threading.Lock()
         self._in_use = 0.0
         self._free_slots = slots
@@ -128,23 +183,30 @@ class Server:
                                 device=self.device) as g:
                 x = torch.empty(g.d * req.nev, dtype=torch.complex128, device=f"cuda:{self.device}")
                 g.fill_gaussian_device(x, req.nev, seed=req.seed, symmetry=1)
-                r = g.gplmr_eig_device(x, req.sigma, req.nev, ns=1, maxit=req.maxit, tol=req.tol,
-                                       symmetry=1, fp32_tol=max(req.tol, 2e-6), inner_iter=req.inner_iter,
+                if self.timing:
+                    g.set_kernel_timing(self.timing)
+                r = g.gplmr_eig_device(x, req.sigma, req.nev, ns=req.ns, maxit=req.maxit, tol=req.tol,
+                                       symmetry=1, fp32_tol=max(req.tol, 1e-5), inner_iter=req.inner_iter,
                                        inner_block=req.inner_block, guard=req.guard)
+                stats = g.kernel_stats() if self.timing else None
+                totals = g.kernel_totals()
+                g.set_kernel_timing(False)
                 res = g.residuals_device(x, r.e)
                 del x
             done = time.time() - t0
             conv = int(np.sum(res <= req.tol))
             return Result(req.rid, True, r.e, res, conv, r.iterations, done - req.arrival,
-                          done - t_admit)
+                          done - t_admit, matvecs=r.matvecs, fp64_restarts=r.fp64_restarts_used,
+                          nn=2 * req.n + 1, nev=req.nev, kernel_stats=stats, kernel_totals=totals)
         except Exception as exc:  # isolate the failure to this request
             done = time.time() - t0
             return Result(req.rid, False, None, None, 0, 0, done - req.arrival, done - t_admit,
                           error=repr(exc))
 
-    def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:
+    def run(self, requests: list[Request], realtime: bool = True, on_result=None) -> list[Result]:
         """Serves the requests (arrival times relative to the start); returns
-        results in request order."""
+        results in request order.  ``on_result(result, request)`` is called as
+        each request completes."""
         pending = sorted(requests)
         results: dict[int, Result] = {}
         threads: list[threading.Thread] = []
@@ -160,7 +222,7 @@ class Server:
             with self._cv:
                 if waiting and self._free_slots > 0:
                     _, _, req = waiting[0]
-                    need = footprint_bytes(req.n, req.nev, req.precision, 1, req.inner_iter,
+                    need = footprint_bytes(req.n, req.nev, req.precision, req.ns, req.inner_iter,
                                            req.inner_block, req.guard)
                     if need > self.cap:
                         heapq.heappop(waiting)
@@ -176,6 +238,8 @@ class Server:
 
                         def work(req=req, need=need, t_admit=t_admit):
                             res = self._solve(req, t_admit, t0)
+                            if on_result is not None:
+                                on_result(res, req)
                             with self._cv:
                                 results[req.rid] = res
                                 self._in_use -= need

@@ -295,6 +295,11 @@ enum {
 };
 int dex_get_kernel_stats(dex_ctx *ctx, int32_t kclass, double *total_ms, int64_t *launches,
                          double *alg_bytes, double *alg_flops);
+/* Per kernel class, every launch since the context was created or the stats
+ * were last reset, timed or not: the launch count and the algorithmic HBM
+ * bytes and flops (host-side counts from the launch shapes; no device call). */
+int dex_get_kernel_totals(const dex_ctx *ctx, int32_t kclass, int64_t *launches, double *alg_bytes,
+                          double *alg_flops);
 int dex_reset_kernel_stats(dex_ctx *ctx);
 
 #ifdef __cplusplus
