@@ -17,7 +17,8 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 BUILDDIR = PKG / "_build"
 ROOT = PKG.parent
-SOURCES = ["dex_setup.cpp", "dex_smallla.cpp", "dex_api.cpp", "dex_eigvecs.cpp", "dex_kernels.hip"]
+SOURCES = ["dex_setup.cpp", "dex_smallla.cpp", "dex_api.cpp", "dex_eigvecs.cpp", "dex_lanczos.cpp",
+           "dex_kernels.hip"]
 ARCH = os.environ.get("DEX_OFFLOAD_ARCH", "gfx950")
 
 

@@ -131,7 +131,25 @@ hipError_t launch_symadd(int nn, int io, const void *x, const void *u, void *y,
 hipError_t launch_symadd_batch(int nn, int io, const void *const *x, const void *const *u, void *const *y,
                                int ncols, const double *vn, const double *tinv, const MatvecCoef &coef,
                                int sign, hipStream_t s);
+// One-electron operator Y = s2 tau3D X + c1 vn o X on ncols columns of NN^3 entries.
+hipError_t launch_h1(const Dbl2 *x, Dbl2 *y, int nn, int ncols, const double *vn, const MatvecCoef &coef,
+                     hipStream_t s);
+// out (p x q) = op(A)^T B over m rows, op = identity (herm 0) or conj (herm 1).
+hipError_t launch_small_dots(const Dbl2 *a, int p, const Dbl2 *b, int q, int64_t m, int herm, Dbl2 *out,
+                             hipStream_t s);
+// Y = beta Y + A C (A m x p, C p x q, Y m x q; beta 0 or 1).
+hipError_t launch_small_lincomb(const Dbl2 *a, int p, const Dbl2 *c, int q, int64_t m, int beta, Dbl2 *y,
+                                hipStream_t s);
 hipError_t launch_to_single(const Dbl2 *x, void *xs, int64_t n, hipStream_t s);
+
+// ---- one-electron block Lanczos (dex_lanczos.cpp) --------------------------
+// BLOCKLANCZOS (lanmod.F90:18-219) of H1 = s2 tau3D + c1 vn on the device:
+// block size bb, at most kdim blocks, a convergence check every `check` steps,
+// numeigs eigenvalues of smallest real part, Hermitian residual tolerance tol.
+// Returns DEX_OK, or DEX_ENOTCONVERGED with the last check's values.
+int heplus_lanczos(hipStream_t stream, int nn, const double *d_vn, const MatvecCoef &coef, int bb, int kdim,
+                   int check, double tol, int numeigs, uint64_t seed, std::vector<std::complex<double>> &eig,
+                   std::vector<std::complex<double>> &vecs, std::vector<double> &resid, int &iters);
 hipError_t launch_to_double(const void *xs, Dbl2 *x, int64_t n, hipStream_t s);
 
 }  // namespace dex

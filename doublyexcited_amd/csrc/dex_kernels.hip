@@ -1133,6 +1133,92 @@ __global__ void k_to_double(const float2 *__restrict__ x, double2 *__restrict__ 
   if (i < n) y[i] = make_double2(double(x[i].x), double(x[i].y));
 }
 
+// ---- one-electron (He+) operator and the small block products of its Lanczos --
+// H1 x = s2 tau3D x + c1 vn o x on vectors of M = NN^3 entries (i fastest):
+// KEMULT1 (hammod.F90:71-134) with the 1/SCALE^2 of the kinetic matrix folded
+// into s2 and POTENTIAL1 (hammod.F90:432-498) / SCALE into c1 vn.  One thread
+// per entry of one column; the column (at most 315 KB at NN = 27) stays in L2.
+__global__ __launch_bounds__(256) void k_h1(const double2 *__restrict__ x, double2 *__restrict__ y, int nn,
+                                            int64_t m, int ncols, const double *__restrict__ vn,
+                                            MatvecCoef cf) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m * ncols) return;
+  const int64_t c = e / m, p = e % m;
+  const int i = int(p % nn), j = int((p / nn) % nn), k = int(p / (int64_t(nn) * nn));
+  const double2 *xc = x + c * m;
+  const int64_t base_i = p - i, base_j = p - int64_t(nn) * j, base_k = p - int64_t(nn) * nn * k;
+  double sr = 0.0, si = 0.0;
+  for (int l = 0; l < nn; ++l) {
+    const double ti = tau_rt(l - i), tj = tau_rt(l - j), tk = tau_rt(l - k);
+    const double2 a = xc[base_i + l], b = xc[base_j + int64_t(nn) * l], d = xc[base_k + int64_t(nn) * nn * l];
+    sr += ti * a.x + tj * b.x + tk * d.x;
+    si += ti * a.y + tj * b.y + tk * d.y;
+  }
+  const double2 xv = xc[p];
+  const double v = vn[p];
+  double2 o;
+  o.x = cf.s2_re * sr - cf.s2_im * si + v * (cf.c1_re * xv.x - cf.c1_im * xv.y);
+  o.y = cf.s2_re * si + cf.s2_im * sr + v * (cf.c1_re * xv.y + cf.c1_im * xv.x);
+  y[e] = o;
+}
+
+// out(i, j) = sum_r op(A_i[r]) B_j[r] over m rows, op = identity (the
+// bilinear product of the complex-symmetric Lanczos, MPIDOT) or conj (the
+// Hermitian one, MPIHERMDOT); A (m x p) and B (m x q) column-major with
+// leading dimension m, out (p x q) column-major.  One workgroup per output
+// element, a fixed-order reduction.
+__global__ __launch_bounds__(256) void k_small_dots(const double2 *__restrict__ a, int p,
+                                                    const double2 *__restrict__ b, int q, int64_t m,
+                                                    int herm, double2 *__restrict__ out) {
+  const int i = blockIdx.x % p, j = blockIdx.x / p;
+  const double2 *ai = a + int64_t(i) * m;
+  const double2 *bj = b + int64_t(j) * m;
+  const double s = herm ? -1.0 : 1.0;
+  double sr = 0.0, si = 0.0;
+  for (int64_t r = threadIdx.x; r < m; r += blockDim.x) {
+    const double2 u = ai[r], v = bj[r];
+    const double uy = s * u.y;
+    sr += u.x * v.x - uy * v.y;
+    si += u.x * v.y + uy * v.x;
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = sr;
+  red[1][threadIdx.x] = si;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (int(threadIdx.x) < off) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + off];
+      red[1][threadIdx.x] += red[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i + int64_t(p) * j] = make_double2(red[0][0], red[1][0]);
+}
+
+// Y = beta Y + A C for A (m x p, leading dimension m), C (p x q, leading
+// dimension p), Y (m x q, leading dimension m); beta 0 or 1.  One thread per
+// element of Y.
+__global__ __launch_bounds__(256) void k_small_lincomb(const double2 *__restrict__ a, int p,
+                                                       const double2 *__restrict__ cm, int q, int64_t m,
+                                                       int beta, double2 *__restrict__ y) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m * q) return;
+  const int64_t r = e % m;
+  const int j = int(e / m);
+  double sr = 0.0, si = 0.0;
+  for (int i = 0; i < p; ++i) {
+    const double2 u = a[r + m * i], c = cm[i + int64_t(p) * j];
+    sr += u.x * c.x - u.y * c.y;
+    si += u.x * c.y + u.y * c.x;
+  }
+  if (beta) {
+    const double2 o = y[e];
+    sr += o.x;
+    si += o.y;
+  }
+  y[e] = make_double2(sr, si);
+}
+
 // Two-electron vector of rank <= K over one-electron orbitals:
 // out(p, q) = sum_b T(p, b) X(q, b), p electron 1 (fastest), q electron 2;
 // T and X are M x K column-major.  One thread per element.
@@ -1381,6 +1467,35 @@ hipError_t launch_to_single(const Dbl2 *x, void *xs, int64_t n, hipStream_t s) {
   const int bs = 256;
   hipLaunchKernelGGL(k_to_single, dim3(unsigned((n + bs - 1) / bs)), dim3(bs), 0, s,
                      reinterpret_cast<const double2 *>(x), static_cast<float2 *>(xs), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_h1(const Dbl2 *x, Dbl2 *y, int nn, int ncols, const double *vn, const MatvecCoef &coef,
+                     hipStream_t s) {
+  const int64_t m = int64_t(nn) * nn * nn;
+  const int64_t n = m * ncols;
+  if (ncols < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_h1, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, reinterpret_cast<const double2 *>(x),
+                     reinterpret_cast<double2 *>(y), nn, m, ncols, vn, coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_small_dots(const Dbl2 *a, int p, const Dbl2 *b, int q, int64_t m, int herm, Dbl2 *out,
+                             hipStream_t s) {
+  if (p < 1 || q < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_small_dots, dim3(unsigned(int64_t(p) * q)), dim3(256), 0, s,
+                     reinterpret_cast<const double2 *>(a), p, reinterpret_cast<const double2 *>(b), q, m, herm,
+                     reinterpret_cast<double2 *>(out));
+  return hipGetLastError();
+}
+
+hipError_t launch_small_lincomb(const Dbl2 *a, int p, const Dbl2 *c, int q, int64_t m, int beta, Dbl2 *y,
+                                hipStream_t s) {
+  if (p < 1 || q < 1) return hipErrorInvalidValue;
+  const int64_t n = m * q;
+  hipLaunchKernelGGL(k_small_lincomb, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const double2 *>(a), p, reinterpret_cast<const double2 *>(c), q, m, beta,
+                     reinterpret_cast<double2 *>(y));
   return hipGetLastError();
 }
 

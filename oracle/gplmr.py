@@ -201,8 +201,13 @@ class GplmrResult:
 
 def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: complex,
               nev: int, ns: int = 1, maxit: int = 100, tol: float = 1e-8,
-              inner_iter: int = 0, callback=None, lock: bool = False) -> GplmrResult:
+              inner_iter: int = 0, callback=None, lock: bool = False, nconv: int | None = None) -> GplmrResult:
     """GPLMR_EIG, gplmr.F90:9-386.
+
+    ``nconv`` (not in the reference; default ``nev``): convergence is tested
+    on the first ``nconv`` columns only, the others being guard columns that
+    the harmonic selection keeps behind the wanted pairs (it orders them
+    nearest first).
 
     ``inner_iter == 1`` applies PRECBLOCK's ``W <- H W`` (gplmr.F90:491-493);
     ``0`` skips the preconditioner (guards at gplmr.F90:92,118,151); ``>= 2``
@@ -240,8 +245,9 @@ def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: c
     ax = matblock(x)
     e = crossmult_diag(x, ax)
     w = ax - x * e[None, :]
+    nconv = nev if nconv is None else nconv
     resnrm = colnorms(w)
-    maxres = resnrm.max()
+    maxres = resnrm[:nconv].max()
     p = np.zeros((dim, 0), np.complex128)
     ap = np.zeros_like(p)
     it = 1
@@ -314,7 +320,7 @@ def gplmr_eig(amul: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, sigma: c
         e = crossmult_diag(x, ax)  # :332
         w = ax - x * e[None, :]  # :334-336
         resnrm = colnorms(w)
-        maxres = resnrm.max()
+        maxres = resnrm[:nconv].max()
         history.append((it, e.copy(), resnrm.copy()))
         if callback is not None:
             callback(it, e, resnrm)
