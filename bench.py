@@ -404,6 +404,38 @@ def served_cpu_baseline(res64, budget_s: float = 25.0) -> dict:
     }
 
 
+def traffic_run_served(out_path: str, rid: int, form: str) -> int:
+    """The program the HBM-traffic counter passes profile for the served
+    stream: a calibration kernel of known size (an elementwise multiply reading
+    and writing 1 GiB), then request ``rid`` of the stream served alone in
+    ``form``; every launch's algorithmic bytes per kernel class (the library's
+    totals) go to ``out_path``."""
+    import torch
+
+    from doublyexcited_amd.serve import Server, poisson_stream
+
+    torch.cuda.set_device(0)
+    n = 64 * 1024 * 1024
+    a = torch.full((n,), 1.0 + 0.5j, dtype=torch.complex128, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
+    torch.cuda.empty_cache()
+    sv = SERVED
+    req = poisson_stream(sv["rate"], rid + 1, sv["seed"], precision=form, plan=True, cap=sv["cap"],
+                         maxit=sv["maxit"], tol=sv["tol"])[rid]
+    req.arrival = 0.0
+    r = Server(sv["cap"], 1, progress=10).run([req], realtime=False)[0]
+    with open(out_path, "w") as fh:
+        json.dump({"request": rid, "form": form, "nn": r.nn, "nev": r.nev, "iterations": r.iterations,
+                   "totals": {k: list(v) for k, v in (r.kernel_totals or {}).items()}}, fh)
+    log(f"traffic run: request {rid} ({form}) NN={r.nn} block {r.nev}: {r.iterations} iterations, "
+        f"{r.converged_pairs} pairs")
+    return 0
+
+
 def run_served(args) -> int:
     import numpy as np
     import torch
@@ -412,7 +444,9 @@ def run_served(args) -> int:
 
     torch.cuda.set_device(0)
     sv = SERVED
-    forms = ["fp64"] if args.no_mixed else ["fp64", "mixed"]
+    forms = ["fp64"] if args.no_mixed else [f for f in args.forms.split(",") if f]
+    if "fp64" not in forms and not args.only_forms:
+        forms.insert(0, "fp64")  # the FP64 form is the reference the mixed one is compared with
     slots = args.concurrency or sv["slots"]
     out = {}
     for form in forms:
@@ -424,7 +458,7 @@ def run_served(args) -> int:
         reqs = poisson_stream(sv["rate"], args.steps, sv["seed"], precision=form, plan=True, cap=sv["cap"],
                               maxit=args.maxit or sv["maxit"], tol=sv["tol"])
         log(f"timed: {len(reqs)} {form} requests, rate {sv['rate']}/s, {slots} slots")
-        srv = Server(sv["cap"], slots, timing=TIMING_SAMPLE)
+        srv = Server(sv["cap"], slots, timing=TIMING_SAMPLE, progress=10)
         torch.cuda.synchronize()
         t0 = time.time()
         res = srv.run(reqs, realtime=True, on_result=lambda r, q: log(
@@ -434,6 +468,13 @@ def run_served(args) -> int:
         torch.cuda.synchronize()
         wall = time.time() - t0
         out[form] = (reqs, res, wall)
+    if "fp64" not in out:  # a probe of the mixed form alone (--only-forms)
+        res, wall = out[forms[0]][1], out[forms[0]][2]
+        print(json.dumps({"form": forms[0], "wall_s": wall, "converged_pairs": sum(r.converged_pairs for r in res),
+                          "latency_s": [r.latency for r in res], "iterations": [r.iterations for r in res],
+                          "roofline": _roofline_from_results(res, TIMING_SAMPLE),
+                          "matvec_roofline_by_grid": _matvec_fraction_by_grid(res)}))
+        return 0
     res64 = out["fp64"][1]
     wall64 = out["fp64"][2]
     lat64 = [r.latency for r in res64 if r.ok]
@@ -516,6 +557,9 @@ def main() -> int:
                     choices=sorted(WORKLOADS) + ["he-served-stream"])
     ap.add_argument("--concurrency", type=int, default=0, help="solves in flight (0: workload default)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--forms", default="fp64,mixed", help="served stream: the forms to serve")
+    ap.add_argument("--only-forms", action="store_true",
+                    help="served stream: serve only --forms (no FP64 reference; a probe, not a bench line)")
     ap.add_argument("--maxit", type=int, default=0,
                     help="served stream: iteration budget per request (0: the workload's, 100)")
     ap.add_argument("--mixed-inner-fp32", action="store_true",
@@ -523,12 +567,16 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-run", metavar="STATS_JSON",
                     help="only run the program the HBM-traffic counter passes profile")
+    ap.add_argument("--traffic-request", type=int, default=6,
+                    help="served stream: the request the traffic run serves (default 6: NN=15, block 8)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: independent solves "
               "would shard across GPUs with no communication, and this build runs on one GPU")
         return 2
     if args.workload == "he-served-stream":
+        if args.traffic_run:
+            return traffic_run_served(args.traffic_run, args.traffic_request, args.forms.split(",")[-1])
         return run_served(args)
     import numpy as np
     import torch

@@ -1677,7 +1677,13 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     if (int st = S.grow(W, nev - W.size())) return st;
     if (int st = residual()) return st;
     maxres = *std::max_element(resn.begin(), resn.begin() + nconv);
-    if (o.verbose && (maxres <= tol || (outmodulus > 0 && iter % outmodulus == 0))) {
+    if (o.verbose == 2 && (maxres <= tol || (outmodulus > 0 && iter % outmodulus == 0))) {
+      int nc = 0;
+      for (int j = 0; j < nconv; ++j) nc += resn[j] <= tol;
+      std::fprintf(stderr, "GPLMR NN=%d block %d%s: iteration %d, %d/%d below %.1e, max residual %.2e\n", c->nn,
+                   nconv, lowp ? " (complex64 search blocks)" : "", iter, nc, nconv, tol, maxres);
+      std::fflush(stderr);
+    } else if (o.verbose && (maxres <= tol || (outmodulus > 0 && iter % outmodulus == 0))) {
       std::printf("ITERATION %5d\n", iter);
       for (int j = 0; j < nev; ++j)
         std::printf("EIG %4d = %20.12f%20.12f,  ERROR = %10.2E\n", j + 1, e[j].real(), e[j].imag(), resn[j]);

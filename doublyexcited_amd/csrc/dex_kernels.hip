@@ -16,6 +16,7 @@
 // real, so the real and imaginary parts are contracted in turn through two
 // LDS buffers of NN^3 reals each (148 KB at NN = 21 in FP64).
 #include <atomic>
+#include <cstdlib>
 #include <mutex>
 
 #include <hip/hip_runtime.h>
@@ -662,22 +663,26 @@ __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
 // workgroup share the output tiles (MT x NT of them) round-robin.
 typedef double dex_v4d __attribute__((ext_vector_type(4)));
 
-// Rows per staged tile: 16, halved while the tile exceeds 48 KB.  16-row tiles
-// (8-33 KB) leave room for 4+ resident workgroups per CU; 32-row tiles
-// measured 10-23% slower at 54-64 x 10 (2 workgroups per CU, VGPR-bound).
-constexpr int gram_mfma_rows(int mt, int nt) {
-  int r = 16;
+// Rows per staged tile: 16 (TALL = 32 for the narrowest shapes), halved while
+// the tile exceeds 48 KB.  16-row tiles (8-33 KB) leave room for 4+ resident
+// workgroups per CU; 32-row tiles measured 10-23% slower at 54-64 x 10 (2
+// workgroups per CU, VGPR-bound).  A Gram of one 16 x 16 output tile (the
+// inner solve's projections of 4-column groups) keeps few bytes in flight per
+// workgroup with 16-row tiles: 32-row tiles measured 5-6% faster there at
+// NN=21 (8 x 4, 16 x 4, 12 x 10) and 5-7% slower at 28 x 4 and 32 x 16.
+constexpr int gram_mfma_rows(int mt, int nt, int tall = 16) {
+  int r = tall;
   while (r > 4 && r * (16 * mt + 16 * nt + 1) * 16 > 48 * 1024) r /= 2;
   return r;
 }
 
-template <int MT, int NT>
+template <int MT, int NT, int TALL = 16>
 __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__restrict__ descs, int kl,
                                                             int kr, int share, int64_t d,
                                                             int64_t rows_per_block,
                                                             double2 *__restrict__ partial) {
   constexpr int LD = 16 * MT + 16 * NT + 1;
-  constexpr int ROWS = gram_mfma_rows(MT, NT);
+  constexpr int ROWS = gram_mfma_rows(MT, NT, TALL);
   constexpr int QMAX = (MT * NT + 3) / 4;
   // loads per thread per tile: every thread issues all of them before any
   // wait, and the next tile's while the matrix cores work on this one
@@ -1299,7 +1304,7 @@ namespace {
 // it: 3 or 4 per CU), capped at kGramBlocks.  A grid of exactly that many runs
 // in one wave of workgroups; 1024 workgroups at 3 per CU would leave a second
 // round on a third of the chip.
-template <int MA, int NB>
+template <int MA, int NB, int TALL>
 int gram_grid() {
   static std::atomic<int> cached{0};
   int grid = cached.load(std::memory_order_acquire);
@@ -1310,8 +1315,8 @@ int gram_grid() {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_gram_mfma<MA, NB>),
-                                                   kGramThreads, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(&k_gram_mfma<MA, NB, TALL>), kGramThreads, 0) != hipSuccess ||
       cus < 1 || per_cu < 1)
     return kGramBlocks;
   grid = cus * per_cu < kGramBlocks ? cus * per_cu : kGramBlocks;
@@ -1319,27 +1324,46 @@ int gram_grid() {
   return grid;
 }
 
+// 32-row tiles for the narrow shapes: on unless DEX_GRAM_ROWS=16 (read once).
+bool gram_tall() {
+  static const bool tall = [] {
+    const char *v = std::getenv("DEX_GRAM_ROWS");
+    return !(v && std::atoi(v) == 16);
+  }();
+  return tall;
+}
+
 template <int MA>
 void gram_grids_nb() {
-  (void)gram_grid<MA, 1>();
-  (void)gram_grid<MA, 2>();
-  (void)gram_grid<MA, 3>();
-  (void)gram_grid<MA, 4>();
-  (void)gram_grid<MA, 5>();
-  (void)gram_grid<MA, 6>();
-  (void)gram_grid<MA, 7>();
-  (void)gram_grid<MA, 8>();
+  (void)gram_grid<MA, 1, 16>();
+  (void)gram_grid<MA, 2, 16>();
+  (void)gram_grid<MA, 3, 16>();
+  (void)gram_grid<MA, 4, 16>();
+  (void)gram_grid<MA, 5, 16>();
+  (void)gram_grid<MA, 6, 16>();
+  (void)gram_grid<MA, 7, 16>();
+  (void)gram_grid<MA, 8, 16>();
+  if constexpr (MA == 1) (void)gram_grid<MA, 1, 32>();
+}
+
+template <int MA, int NB, int TALL>
+int gram_launch_t(const ColDesc *descs, int kl, int kr, int share, int64_t d, double2 *partial,
+                  hipStream_t s) {
+  const int grid = gram_grid<MA, NB, TALL>();
+  int64_t rpb = (d + grid - 1) / grid;
+  rpb = (rpb + 63) / 64 * 64;  // a whole number of tiles for every tile height
+  hipLaunchKernelGGL((k_gram_mfma<MA, NB, TALL>), dim3(grid), dim3(kGramThreads), 0, s, descs, kl, kr, share,
+                     d, rpb, partial);
+  return grid;
 }
 
 template <int MA, int NB>
 int gram_launch(const ColDesc *descs, int kl, int kr, int share, int64_t d, double2 *partial,
                 hipStream_t s) {
-  const int grid = gram_grid<MA, NB>();
-  int64_t rpb = (d + grid - 1) / grid;
-  rpb = (rpb + 63) / 64 * 64;  // a whole number of tiles for every tile height
-  hipLaunchKernelGGL((k_gram_mfma<MA, NB>), dim3(grid), dim3(kGramThreads), 0, s, descs, kl, kr, share, d,
-                     rpb, partial);
-  return grid;
+  if constexpr (MA == 1 && NB == 1) {
+    if (gram_tall()) return gram_launch_t<MA, NB, 32>(descs, kl, kr, share, d, partial, s);
+  }
+  return gram_launch_t<MA, NB, 16>(descs, kl, kr, share, d, partial, s);
 }
 
 template <int MA>

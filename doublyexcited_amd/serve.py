@@ -159,13 +159,16 @@ def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),
 class Server:
     """``timing`` > 0 brackets a random ~1 in ``timing`` launches of each
     solve's kernels with HIP events (1: every launch) and returns the per-class
-    statistics with the result."""
+    statistics with the result; ``progress`` > 0 has each solve report on stderr
+    every ``progress`` iterations."""
 
-    def __init__(self, hbm_cap_bytes: float = 240e9, slots: int = 4, device: int = 0, timing: int = 0):
+    def __init__(self, hbm_cap_bytes: float = 240e9, slots: int = 4, device: int = 0, timing: int = 0,
+                 progress: int = 0):
         self.cap = float(hbm_cap_bytes)
         self.slots = slots
         self.device = device
         self.timing = int(timing)
+        self.progress = int(progress)  # > 0: a progress line to stderr every this many iterations
         self._lock = threading.Lock()
         self._in_use = 0.0
         self._free_slots = slots
@@ -187,7 +190,8 @@ class Server:
                     g.set_kernel_timing(self.timing)
                 r = g.gplmr_eig_device(x, req.sigma, req.nev, ns=req.ns, maxit=req.maxit, tol=req.tol,
                                        symmetry=1, fp32_tol=max(req.tol, 1e-5), inner_iter=req.inner_iter,
-                                       inner_block=req.inner_block, guard=req.guard)
+                                       inner_block=req.inner_block, guard=req.guard,
+                                       verbose=2 if self.progress else False, outmodulus=self.progress)
                 stats = g.kernel_stats() if self.timing else None
                 totals = g.kernel_totals()
                 g.set_kernel_timing(False)

@@ -131,7 +131,8 @@ typedef struct dex_result {
 
 typedef struct dex_solve_opts {
   int32_t strict_cholesky; /* 1: fail with DEX_ECHOLESKY where the reference STOPs */
-  int32_t verbose;         /* print "EIG j = ..." every OUTMODULUS iterations like gplmr.F90:343-352 */
+  int32_t verbose;         /* 1: print "EIG j = ..." every OUTMODULUS iterations like gplmr.F90:343-352;
+                              2: one progress line to stderr every OUTMODULUS iterations instead */
   int32_t refresh_every;   /* >0: recompute AX = H X every this many iterations (0 = never, as the reference) */
   int32_t fp64_restarts;   /* FP32 contexts: at most this many restarts from an AX = H X recomputed
                               in FP64 after the first run (a restart that starts below TOL ends them) */
