@@ -456,7 +456,8 @@ def run_served(args) -> int:
                     for i in range(args.warmup)]
             Server(sv["cap"], slots).run(warm, realtime=False)
         reqs = poisson_stream(sv["rate"], args.steps, sv["seed"], precision=form, plan=True, cap=sv["cap"],
-                              maxit=args.maxit or sv["maxit"], tol=sv["tol"])
+                              maxit=args.maxit or sv["maxit"], tol=sv["tol"],
+                              plan_precision="fp64" if args.same_plan else None)
         log(f"timed: {len(reqs)} {form} requests, rate {sv['rate']}/s, {slots} slots")
         srv = Server(sv["cap"], slots, timing=TIMING_SAMPLE, progress=10)
         torch.cuda.synchronize()
@@ -560,6 +561,8 @@ def main() -> int:
     ap.add_argument("--forms", default="fp64,mixed", help="served stream: the forms to serve")
     ap.add_argument("--only-forms", action="store_true",
                     help="served stream: serve only --forms (no FP64 reference; a probe, not a bench line)")
+    ap.add_argument("--same-plan", action="store_true",
+                    help="served stream: both forms use the FP64 form's solver settings (NS, inner solve)")
     ap.add_argument("--maxit", type=int, default=0,
                     help="served stream: iteration budget per request (0: the workload's, 100)")
     ap.add_argument("--mixed-inner-fp32", action="store_true",

@@ -35,7 +35,7 @@ EXPORTS = [
     "dex_dots",
     "dex_dots_device", "dex_solve_opts_default", "dex_gplmr_eig", "dex_gplmr_eig_device",
     "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint",
-    "dex_solve_footprint", "dex_precblock_device", "dex_diagsubspace_device",
+    "dex_solve_footprint", "dex_precblock_device", "dex_diagsubspace_device", "dex_heplus_lanczos_device",
     "dex_status_string", "dex_last_error", "dex_set_kernel_timing", "dex_get_kernel_timing",
     "dex_get_kernel_stats", "dex_get_kernel_totals", "dex_reset_kernel_stats",
     "dex_host_kematrix", "dex_host_getinverse", "dex_host_solve_gevp",
@@ -117,6 +117,8 @@ def load() -> C.CDLL:
                                               C.c_int32, C.c_void_p, dp, dp]),
         "dex_precblock_device": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                            C.c_double, C.c_double, C.c_int32, C.c_int32]),
+        "dex_heplus_lanczos_device": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                                C.c_uint64, dp, dp, dp, C.POINTER(C.c_int32)]),
         "dex_status_string": (C.c_char_p, [C.c_int]),
         "dex_last_error": (C.c_char_p, []),
         "dex_set_kernel_timing": (C.c_int, [P, C.c_int32]),

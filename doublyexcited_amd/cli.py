@@ -197,8 +197,13 @@ def run(cfg: dict, out=sys.stdout, workdir: str | Path = ".") -> dict:
                 ex[j] = complex(torch.vdot(xj, y).item() / torch.vdot(xj, xj).item())
         else:
             numskip = int(cfg["NUMSKIP"])
+            # DIAGONALIZE -> BLOCKLANCZOS (doublyexcited.F90:138, utils.F90:219) on the device,
+            # to the deck's LANTOL at most (the product start vectors need no more)
+            print("CALLING LANCZOS SOLVER...", file=out)
             _, orb = heplus_orbitals(n, delta, theta, int(cfg["NUMORBS"]), charge=int(cfg["CHARGE"]),
-                                     nsmall=int(cfg["NSMALL"]), tfactor=int(cfg["TFACTOR"]))
+                                     nsmall=int(cfg["NSMALL"]), tfactor=int(cfg["TFACTOR"]), method="lanczos",
+                                     blocksize=int(cfg["LANBLOCKSIZE"]), kdim=int(cfg["KDIM"]),
+                                     check=int(cfg["CHECK"]), lantol=min(float(cfg["LANTOL"]), 1e-8))
             print("COMPUTING INITIAL GUESSES...", file=out)
             psi, e0, ex = g.diagsubspace_device(orb, nev + numskip, complex(cfg["GPLMR_SHIFT"]),
                                                 singlet=singlet, hermopt=bool(cfg["HERMOPT"]))

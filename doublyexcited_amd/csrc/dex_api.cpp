@@ -510,7 +510,7 @@ void dex_params_default(dex_params *p) {
 void dex_solve_opts_default(dex_solve_opts *o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
-  o->fp64_restarts = 2;
+  o->fp64_restarts = 0;
   o->fp32_tol = 1e-5;
 }
 
@@ -1508,10 +1508,43 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
   int iter = 1;
   Block Sblk, ASblk;
   Sblk.fmt = ASblk.fmt = lf;
-  while (iter <= maxit && maxres > tol) {
-    if (o.refresh_every > 0 && iter > 1 && (iter - 1) % o.refresh_every == 0) {
-      if (int st = S.matblock(X, AX)) return st;
+  // With complex64 search blocks (lowp) AX is carried by a recurrence over FP32
+  // images, which leaves it off H X by the FP32 rounding of the steps since AX
+  // was last formed.  AX = H X is recomputed in FP64 every `refresh` iterations
+  // (refresh_every, 10 by default in that form), once when the residual first
+  // falls below fp32_tol, and whenever the residual falls below TOL: the run
+  // ends only on a residual formed from an FP64 AX.
+  const int refresh = o.refresh_every > 0 ? o.refresh_every : (lowp ? 10 : 0);
+  bool fresh = true;         // AX was formed by a matvec since the last update
+  bool crossed = false;      // the residual has fallen below fp32_tol once
+  int refreshes = 0;
+  auto refresh_ax = [&]() -> int {
+    if (int st = S.matblock(X, AX)) return st;
+    if (int st = residual()) return st;
+    maxres = *std::max_element(resn.begin(), resn.begin() + nconv);
+    fresh = true;
+    ++refreshes;
+    return 0;
+  };
+  while (iter <= maxit && (maxres > tol || !fresh)) {
+    if (lowp && maxres <= tol && !fresh) {
+      // converged on the recurrence: check against an FP64 AX
+      if (int st = refresh_ax()) return st;
+      if (maxres <= tol) break;
     }
+    if (refresh > 0 && iter > 1 && (iter - 1) % refresh == 0) {
+      if (!lowp) {
+        if (int st = S.matblock(X, AX)) return st;
+      } else if (!fresh) {
+        if (int st = refresh_ax()) return st;
+      }
+    }
+    if (lowp && !crossed && maxres <= o.fp32_tol) {
+      crossed = true;
+      if (!fresh)
+        if (int st = refresh_ax()) return st;
+    }
+    fresh = !lowp;  // in FP64 the recurrence is the reference's: its residual is accepted
     // active columns: all, or with lock only those still above TOL
     std::vector<int> active;
     for (int j = 0; j < nev; ++j)
@@ -1691,10 +1724,15 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     }
     ++iter;
   }
+  if (lowp && !fresh) {
+    // the iteration budget ran out on a recurrence residual: report the true one
+    if (int st = refresh_ax()) return st;
+  }
   for (int j = 0; j < nconv; ++j) e_out[j] = e[j];
   if (res) {
     res->iterations = iter - 1;
     res->converged = maxres <= tol;
+    res->fp64_restarts_used += refreshes;
     res->matvecs += S.matvecs;
     res->max_residual = maxres;
     res->t_total += now_s() - t_start;
@@ -1822,15 +1860,11 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   std::vector<cd> ev(nev);
   int st;
   // Mixed form (FP32 contexts): complex64 search blocks with FP32 matvecs
-  // around the complex128 X and AX.  AX is carried by the recurrence of the
-  // block updates, so the FP32 images leave it off H X by about the FP32
-  // rounding of the early steps; the first run therefore stops at
-  // max(tol, fp32_tol), and each restart recomputes AX = H X in FP64, so the
-  // residual it starts from (and tests) is the true one, and refines to tol.
-  const double tol1 = mixed ? std::max(tol, o.fp32_tol) : tol;
-  st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol1, inner_iter, outmodulus, o, res, mixed,
-                 store);
-  if (mixed && (st == DEX_OK || st == DEX_ENOTCONVERGED)) {
+  // around the complex128 X and AX, whose FP64 refreshes gplmr_run does; a run
+  // that ends at MAXIT unconverged may be followed by up to fp64_restarts
+  // restarts from its X (each with a fresh FP64 AX and no P block).
+  st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol, inner_iter, outmodulus, o, res, mixed, store);
+  if (mixed && st == DEX_ENOTCONVERGED) {
     for (int r = 0; r < o.fp64_restarts; ++r) {
       dex_result sub{};
       st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol, inner_iter, outmodulus, o,
@@ -1847,11 +1881,9 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
         res->fallbacks += sub.fallbacks;
         res->max_residual = sub.max_residual;
         res->converged = sub.converged;
-        res->fp64_restarts_used = r + 1;
+        res->fp64_restarts_used += sub.fp64_restarts_used;
       }
-      // a restart that starts converged (true residual <= tol) ends the refinement
-      if (st != DEX_ENOTCONVERGED && sub.iterations == 0) break;
-      if (st != DEX_OK && st != DEX_ENOTCONVERGED) break;
+      if (st != DEX_ENOTCONVERGED) break;
     }
   }
   free_all();
@@ -2084,6 +2116,27 @@ extern "C" int dex_gplmr_eig(dex_ctx *c, double *x, double *e, int32_t m, int32_
   err = hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost);
   (void)hipFree(dx);
   if (err != hipSuccess && st == DEX_OK) return hip_fail(err, "copy out");
+  return st;
+}
+
+extern "C" int dex_heplus_lanczos_device(dex_ctx *c, int32_t blocksize, int32_t kdim, int32_t check, double tol,
+                                         int32_t numeigs, uint64_t seed, double *eigenvalues, double *orbitals,
+                                         double *residuals, int32_t *iterations) {
+  if (!c || !eigenvalues) return fail(DEX_EINVAL, "null argument");
+  DEX_HIP(hipSetDevice(c->p.device));
+  std::vector<cd> eig, vecs;
+  std::vector<double> resid;
+  int iters = 0;
+  const int st = dex::heplus_lanczos(c->stream, c->nn, c->d_vn, c->coef, blocksize, kdim, check, tol, numeigs,
+                                     seed, eig, vecs, resid, iters);
+  if (st != DEX_OK && st != DEX_ENOTCONVERGED) return st;
+  for (int k = 0; k < numeigs && k < int(eig.size()); ++k) {
+    eigenvalues[2 * k] = eig[k].real();
+    eigenvalues[2 * k + 1] = eig[k].imag();
+    if (residuals) residuals[k] = resid[k];
+  }
+  if (orbitals && !vecs.empty()) std::memcpy(orbitals, vecs.data(), vecs.size() * sizeof(cd));
+  if (iterations) *iterations = iters;
   return st;
 }
 

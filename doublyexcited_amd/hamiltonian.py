@@ -262,6 +262,23 @@ class HeliumGrid:
                                                 ex.view(np.float64).ctypes.data_as(_dp)))
         return psi, e0, ex
 
+    def heplus_lanczos(self, numeigs: int, blocksize: int = 3, kdim: int = 501, check: int = 10,
+                       tol: float = 1e-8, seed: int = 1, allow_unconverged: bool = False):
+        """BLOCKLANCZOS (lanmod.F90:18-219) of the one-electron operator on the
+        device.  Returns (eigenvalues (numeigs), orbitals (M x numeigs),
+        residuals (numeigs), iterations)."""
+        kdim = min(kdim, self.m // blocksize)
+        e = np.zeros(numeigs, dtype=np.complex128)
+        flat = np.zeros(self.m * numeigs, dtype=np.complex128)  # column-major M x numeigs
+        res = np.zeros(numeigs)
+        it = C.c_int32()
+        st = self._lib.dex_heplus_lanczos_device(self._ctx, blocksize, kdim, check, float(tol), numeigs, seed,
+                                                 e.view(np.float64).ctypes.data_as(_dp),
+                                                 flat.view(np.float64).ctypes.data_as(_dp), _ptr(res),
+                                                 C.byref(it))
+        _lib.check(st, allow=(_lib.DEX_OK, _lib.DEX_ENOTCONVERGED) if allow_unconverged else (_lib.DEX_OK,))
+        return e, flat.reshape(self.m, numeigs, order="F"), res, it.value
+
     def footprint(self, nev: int, ns: int = 1, inner_iter: int = 0, inner_block: int = 0,
                   guard: int = 0, inner_fp32: bool = False) -> int:
         o = self._opts(False, False, 0, 2, 1e-5, 0, False, inner_block, guard, inner_fp32)
@@ -295,7 +312,7 @@ class HeliumGrid:
 
     def gplmr_eig(self, x: np.ndarray, sigma: complex, nev: int, ns: int = 1, maxit: int = 100,
                   tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5, strict: bool = False,
-                  verbose: bool = False, refresh_every: int = 0, fp64_restarts: int = 2,
+                  verbose: bool = False, refresh_every: int = 0, fp64_restarts: int = 0,
                   fp32_tol: float = 1e-5, m: int | None = None, mb: int | None = None,
                   symmetry: int = 0, lock: bool = False, inner_block: int = 0,
                   guard: int = 0, inner_fp32: bool = False) -> SolveResult:
@@ -319,7 +336,7 @@ class HeliumGrid:
     def gplmr_eig_device(self, x, sigma: complex, nev: int, ns: int = 1, maxit: int = 100,
                          tol: float = 1e-12, inner_iter: int = 0, outmodulus: int = 5,
                          strict: bool = False, verbose: bool = False, refresh_every: int = 0,
-                         fp64_restarts: int = 2, fp32_tol: float = 1e-5,
+                         fp64_restarts: int = 0, fp32_tol: float = 1e-5,
                          symmetry: int = 0, lock: bool = False, inner_block: int = 0,
                          guard: int = 0, inner_fp32: bool = False) -> SolveResult:
         """GPLMR_EIG with the block resident on the device (torch complex128 tensor)."""

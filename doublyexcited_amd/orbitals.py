@@ -1,12 +1,13 @@
 """One-electron (He+) orbitals for the orbital-product start vectors.
 
 The reference obtains them with a block Lanczos on the one-electron operator
-(lanmod.F90:18-219, KEMULT1/POTENTIAL1, hammod.F90:71-134,432-498).  Here the
-one-electron Hamiltonian H1 = T3D / SCALE^2 + Vn / SCALE (SCALE = e^{i theta};
-initmod.F90:204-224) is built on the host from the library's own tables
-(``dex_host_kematrix``, ``dex_host_getinverse``) and diagonalised densely with
-LAPACK, which is practical up to NN = 13 (M = 2197).  The lowest ``numorbs``
-eigenpairs by real part are returned, as the Lanczos delivers them.
+(lanmod.F90:18-219, KEMULT1/POTENTIAL1, hammod.F90:71-134,432-498), which
+``heplus_orbitals`` runs on the device (``dex_heplus_lanczos_device``) for any
+grid.  ``method="dense"`` instead builds H1 = T3D / SCALE^2 + Vn / SCALE
+(SCALE = e^{i theta}; initmod.F90:204-224) on the host from the library's own
+tables (``dex_host_kematrix``, ``dex_host_getinverse``) and diagonalises it
+with LAPACK, practical up to NN = 13 (M = 2197).  Either way the ``numorbs``
+eigenpairs of smallest real part are returned, as the Lanczos delivers them.
 """
 from __future__ import annotations
 
@@ -44,13 +45,27 @@ def one_electron_hamiltonian(n: int, delta: float, theta: float, charge: int = 2
 
 
 def heplus_orbitals(n: int, delta: float, theta: float, numorbs: int, charge: int = 2, nsmall: int = 22,
-                    tfactor: int = 4):
-    """(eigenvalues (numorbs), orbitals M x numorbs), lowest real parts first."""
+                    tfactor: int = 4, method: str = "lanczos", blocksize: int = 3, kdim: int = 501,
+                    check: int = 10, lantol: float = 1e-8, seed: int = 1):
+    """(eigenvalues (numorbs), orbitals M x numorbs), lowest real parts first.
+
+    ``method="lanczos"``: BLOCKLANCZOS on the device with LANBLOCKSIZE
+    ``blocksize``, KDIM ``kdim``, CHECK ``check`` and LANTOL ``lantol``;
+    ``"dense"``: the host eigendecomposition (NN <= 13)."""
+    if method == "lanczos":
+        from .hamiltonian import HeliumGrid
+
+        with HeliumGrid(n, delta, theta, charge=charge, nsmall=nsmall, tfactor=tfactor) as g:
+            e, orb, _, _ = g.heplus_lanczos(numorbs, blocksize=blocksize, kdim=kdim, check=check, tol=lantol,
+                                            seed=seed)
+        return e, orb
+    if method != "dense":
+        raise ValueError(f"unknown method {method!r}")
     m = (2 * n + 1) ** 3
     if m > MAX_DENSE_M:
         raise DexError(_lib.DEX_EUNSUPPORTED,
-                       f"dense He+ orbitals are built up to NN=13 (M={MAX_DENSE_M}); start from an "
-                       "eigenvector file (Read / Infile=) on larger grids")
+                       f"dense He+ orbitals are built up to NN=13 (M={MAX_DENSE_M}); use the device "
+                       "block Lanczos (method='lanczos') on larger grids")
     w, v = np.linalg.eig(one_electron_hamiltonian(n, delta, theta, charge, nsmall, tfactor))
     o = np.argsort(w.real, kind="stable")[:numorbs]
     return w[o], v[:, o]

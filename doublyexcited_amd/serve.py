@@ -134,10 +134,11 @@ def plan_solve(n: int, nev: int, precision: str, cap: float = 240e9) -> dict:
 def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),
                    delta_range=(0.35, 0.6), theta_range=(0.2, 0.5),
                    sigma_re=(-0.78, -0.69, -0.60), sigma_im=-0.002, nev_choices=(4, 8, 16),
-                   tol=1e-7, precision="fp64", plan: bool = False, cap: float = 240e9, maxit: int = 3000):
+                   tol=1e-7, precision="fp64", plan: bool = False, cap: float = 240e9, maxit: int = 3000,
+                   plan_precision: str | None = None):
     """Seeded request stream with Poisson arrivals (rate per second).  With
     ``plan`` each request carries the solver settings of ``plan_solve`` for
-    its precision and the cap."""
+    its precision (or ``plan_precision``) and the cap."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -150,7 +151,7 @@ def poisson_stream(rate: float, count: int, seed: int, nn_choices=range(11, 22),
                     nev=int(rng.choice(nev_choices)), tol=tol, precision=precision,
                     seed=seed * 100003 + i, maxit=maxit)
         if plan:
-            for k, v in plan_solve(r.n, r.nev, precision, cap).items():
+            for k, v in plan_solve(r.n, r.nev, plan_precision or precision, cap).items():
                 setattr(r, k, v)
         out.append(r)
     return out

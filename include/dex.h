@@ -126,18 +126,19 @@ typedef struct dex_result {
   double t_update;        /* block recombinations (s) */
   double t_smalleig;      /* host projected pencil (s) */
   int32_t fallbacks;      /* orthonormalisations that left the plain cholQR path */
-  int32_t fp64_restarts_used; /* FP32 contexts: FP64 restarts that ran */
+  int32_t fp64_restarts_used; /* FP32 contexts: recomputations of AX = H X in FP64 (refreshes) */
 } dex_result;
 
 typedef struct dex_solve_opts {
   int32_t strict_cholesky; /* 1: fail with DEX_ECHOLESKY where the reference STOPs */
   int32_t verbose;         /* 1: print "EIG j = ..." every OUTMODULUS iterations like gplmr.F90:343-352;
                               2: one progress line to stderr every OUTMODULUS iterations instead */
-  int32_t refresh_every;   /* >0: recompute AX = H X every this many iterations (0 = never, as the reference) */
-  int32_t fp64_restarts;   /* FP32 contexts: at most this many restarts from an AX = H X recomputed
-                              in FP64 after the first run (a restart that starts below TOL ends them) */
-  double fp32_tol;         /* FP32 contexts: residual target of the first run, whose AX is carried by
-                              the recurrence over FP32 images (default 1e-5) */
+  int32_t refresh_every;   /* >0: recompute AX = H X every this many iterations; 0: never in FP64
+                              contexts (as the reference), every 10 in FP32 contexts */
+  int32_t fp64_restarts;   /* FP32 contexts: at most this many restarts (fresh FP64 AX, no P block) after
+                              a run that ends at MAXIT unconverged (default 0) */
+  double fp32_tol;         /* FP32 contexts: AX is also recomputed in FP64 when the residual first falls
+                              below this (default 1e-5), where the FP32 drift starts to matter */
   int32_t symmetry;        /* 0: general HAMMULT; +1 / -1: project the start block to
                               singlet / triplet and use dex_matvec_sym_device */
   int32_t lock;            /* 1: build W, S, P only for columns whose residual is above
@@ -180,9 +181,11 @@ int dex_gplmr_eig(dex_ctx *ctx, double *x, double *e, int32_t m, int32_t mb,
  * applied in FP64, while the search blocks W, S, P, their images and the inner
  * solve's Arnoldi basis are stored in complex64 and multiplied by H in FP32
  * (FP32 accumulation); every Gram, combination, Cholesky factor and the
- * projected pencil are FP64.  The first run stops at max(TOL, fp32_tol); each of
- * up to fp64_restarts restarts recomputes AX = H X in FP64 and iterates to TOL.
- * The reference is FP64 throughout (definitions.INC:1-2). */
+ * projected pencil are FP64.  AX, carried by the recurrence over FP32 images,
+ * is recomputed as H X in FP64 every refresh_every iterations (10 by default),
+ * when the residual first falls below fp32_tol, and whenever it falls below
+ * TOL: the solve ends only on a residual formed from an FP64 AX.  The
+ * reference is FP64 throughout (definitions.INC:1-2). */
 int dex_gplmr_eig_device(dex_ctx *ctx, void *x_dev, double *e, int32_t m, int32_t mb,
                          double sigma_re, double sigma_im, int32_t nev, int32_t ns,
                          int32_t maxit, double tol, int32_t inner_iter, int32_t outmodulus,
@@ -224,6 +227,24 @@ int dex_precblock_device(dex_ctx *ctx, const void *x_dev, void *px_dev, int32_t 
 int dex_diagsubspace_device(dex_ctx *ctx, const double *orbitals, int32_t numorbs, int32_t singlet,
                             int32_t hermopt, double etarget_re, double etarget_im, int32_t numout,
                             void *psi_dev, double *e0, double *expects);
+
+/* BLOCKLANCZOS(PARAMS,LANPAR,HAM,MPIDATA,BB), lanmod.F90:18-219, on the
+ * device: the complex-symmetric block Lanczos of the one-electron (He+)
+ * operator H1 = T3D / SCALE^2 + Vn / SCALE of this context's grid (KEMULT1 and
+ * POTENTIAL1, hammod.F90:71-134,432-498), whose eigenvectors are the orbitals
+ * of the product start vectors (dex_diagsubspace_device).  blocksize is
+ * LANBLOCKSIZE, kdim KDIM (blocks kept; blocksize * kdim <= NN^3), check CHECK
+ * (a convergence test every this many steps), tol LANTOL on the Hermitian
+ * residual norms, numeigs NUMORBS (the eigenvalues of smallest real part).
+ * The start block is seeded (uniform [0,1) real parts, 1e-4 i uniform
+ * imaginary parts, as RANDOM_NUMBER fills it).  Out: eigenvalues (numeigs
+ * complex), orbitals (M x numeigs complex, column-major, Hermitian-normalised;
+ * may be NULL), residuals (numeigs; may be NULL), iterations (may be NULL).
+ * DEX_ENOTCONVERGED ("LANCZOS FAILED TO CONVERGE") returns the last check's
+ * values. */
+int dex_heplus_lanczos_device(dex_ctx *ctx, int32_t blocksize, int32_t kdim, int32_t check, double tol,
+                              int32_t numeigs, uint64_t seed, double *eigenvalues, double *orbitals,
+                              double *residuals, int32_t *iterations);
 
 /* Bytes of device memory a dex_gplmr_eig_device solve of (nev, ns, inner_iter,
  * opts: inner_block, guard and inner_fp32; NULL for the defaults) holds on this

@@ -38,7 +38,7 @@ def test_status_strings_and_defaults():
     assert (p.n, p.delta, p.charge, p.nsmall, p.tfactor) == (2, 0.5, 2, 22, 4)
     o = _lib.DexSolveOpts()
     lib.dex_solve_opts_default(C.byref(o))
-    assert o.fp64_restarts == 2 and o.strict_cholesky == 0
+    assert o.fp64_restarts == 0 and o.strict_cholesky == 0 and o.fp32_tol == 1e-5 and o.refresh_every == 0
 
 
 def test_host_kematrix_matches_oracle():

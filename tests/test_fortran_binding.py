@@ -21,7 +21,7 @@ def test_fortran_binding_compiles_links_and_runs(tmp_path):
     subprocess.run([FLANG, "-O1", "-J", str(tmp_path), str(src), "-o", str(exe), f"-L{libdir}", "-ldex",
                     f"-Wl,-rpath,{libdir}"], check=True, cwd=tmp_path, timeout=300)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=60).stdout.split()
-    assert out[:3] == ["2", "22", "2"]
+    assert out[:3] == ["2", "22", "0"]  # dex_params_default N, NSMALL; dex_solve_opts_default fp64_restarts
     t = kematrix(5, 0.5)
     assert float(out[3]) == pytest.approx(t[0, 0], rel=1e-15)
     assert float(out[4]) == pytest.approx(t[1, 0], rel=1e-15)

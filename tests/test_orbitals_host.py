@@ -19,7 +19,7 @@ def test_one_electron_hamiltonian_equals_oracle():
 def test_heplus_orbitals_reproduce_reference_log(golden_dir):
     g = json.loads((golden_dir / "heplus_nn11_d2_t0.2.json").read_text())
     logv = np.array([complex(a, b) for a, b in g["heplus_eigenvalues"]])
-    w, v = heplus_orbitals(5, 2.0, 0.2, 14)
+    w, v = heplus_orbitals(5, 2.0, 0.2, 14, method="dense")
     # converged levels to 1e-12; the log's LANTOL 1e-2 members agree to its own spread
     assert abs(w[0] - logv[0]) < 1e-12 and abs(w[1] - logv[1]) < 1e-12
     assert np.max(np.abs(w - logv)) < 1e-5
