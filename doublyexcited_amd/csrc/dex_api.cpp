@@ -51,6 +51,35 @@ namespace dex {
 int set_error(int status, const char *msg) { return fail(status, msg); }
 }  // namespace dex
 
+// ---- device scratch for the allocation-free matvec ---------------------------
+// dex_matvec_noalloc keeps the matvec intermediate U in this file-scope device
+// array instead of a library allocation.  Every GPU the library runs on holds
+// its own copy (64 MiB) until the process ends.  Host code never passes the
+// array's name to a kernel: it is a host-side stand-in; the device address of
+// the current device's copy comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current device.
+hipError_t scratch_address(int device, void **ptr) {
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 struct dex_ctx {
   dex_params p{};
   int nn = 0;
@@ -62,6 +91,7 @@ struct dex_ctx {
   bool fast64 = false, fast32 = false;
   void *d_u = nullptr;  // matvec intermediate, D complex128 (also used as D complex64)
   void *d_ub = nullptr;  // intermediates of a batched matvec launch: batch_cols(c) x D complex128
+  void *d_scratch = nullptr;  // this device's copy of `scratch` (dex_matvec_noalloc)
   ColDesc *d_desc = nullptr;
   unsigned char *d_args = nullptr;  // a combine's descriptors, C and beta, packed (one upload)
   Dbl2 *d_gram_out = nullptr;
@@ -607,6 +637,8 @@ int dex_ctx_create(const dex_params *p, dex_ctx **out) {
     c->stage_cap = size_t(4) << 20;
     e = hipHostMalloc(reinterpret_cast<void **>(&c->h_stage), c->stage_cap, hipHostMallocDefault);
   }
+  // the allocation-free matvec's scratch on this device (the device is current)
+  if (e == hipSuccess) e = scratch_address(p->device, &c->d_scratch);
   if (e != hipSuccess) {
     dex_ctx_destroy(c);
     return hip_fail(e, "dex_ctx_create");
@@ -771,6 +803,43 @@ int dex_matvec_sym_device(dex_ctx *c, const void *in, void *out, int32_t howmany
     DEX_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
     c->t_pass1 = a;
     c->t_pass2 = b;
+  }
+  return DEX_OK;
+}
+
+int dex_matvec_noalloc(dex_ctx *c, const void *in, void *out, int32_t howmany, int64_t checksize,
+                       int32_t single, int32_t symmetry) {
+  if (symmetry < -1 || symmetry > 1) return fail(DEX_EINVAL, "symmetry must be -1, 0 or 1");
+  if (!c || !in || !out) return fail(DEX_EINVAL, "null argument");
+  if (checksize != c->d) return fail(DEX_ESIZE, "SIZE ERROR: checksize != M*MBLK");
+  if (howmany < 0) return fail(DEX_EINVAL, "howmany < 0");
+  const bool sp = single != 0;
+  if (sp && c->p.precision != DEX_FP32) return fail(DEX_EINVAL, "complex64 I/O needs an FP32 context");
+  const bool f32 = c->p.precision == DEX_FP32;  // FP32 arithmetic (complex64 U)
+  const bool fast = f32 ? c->fast32 : c->fast64;
+  if (!fast && (f32 || symmetry != 0))
+    return fail(DEX_EUNSUPPORTED, "this grid and precision have no LDS matvec kernels");
+  if (fast && size_t(c->d) * (f32 ? 8 : 16) > kScratchBytes)
+    return fail(DEX_ESIZE, "dex_matvec_noalloc: the intermediate (NN^6 complex128, or complex64 in an FP32 "
+                           "context) exceeds the 64 MiB device scratch: NN <= 11 in FP64, NN <= 13 in FP32");
+  if (!c->d_scratch) return fail(DEX_EHIP, "no device scratch address");
+  DEX_HIP(hipSetDevice(c->p.device));
+  const int io = sp ? dex::kIoF32 : (f32 ? dex::kIoMixed : dex::kIoF64);
+  const size_t xb = sp ? 8 : 16;  // bytes per entry of X and Y
+  for (int j = 0; j < howmany; ++j) {
+    const char *x = static_cast<const char *>(in) + size_t(j) * size_t(c->d) * xb;
+    char *y = static_cast<char *>(out) + size_t(j) * size_t(c->d) * xb;
+    if (!fast) {
+      DEX_HIP(dex::launch_matvec_generic(c->nn, reinterpret_cast<const Dbl2 *>(x), reinterpret_cast<Dbl2 *>(y),
+                                         c->d_vn, c->d_tinv, c->coef, c->stream));
+      continue;
+    }
+    // U (fully written by the electron-1 pass before the second pass reads it)
+    DEX_HIP(dex::launch_pass1(c->nn, io, x, c->d_scratch, c->stream));
+    if (symmetry != 0)
+      DEX_HIP(dex::launch_symadd(c->nn, io, x, c->d_scratch, y, c->d_vn, c->d_tinv, c->coef, symmetry, c->stream));
+    else
+      DEX_HIP(dex::launch_pass2(c->nn, io, x, c->d_scratch, y, c->d_vn, c->d_tinv, c->coef, c->stream));
   }
   return DEX_OK;
 }

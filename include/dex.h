@@ -105,6 +105,20 @@ int dex_matvec_device(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
 int dex_matvec_sym_device(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
                           int64_t checksize, int32_t single, int32_t symmetry);
 
+/* dex_matvec_sym_device without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops that forbid it).  The intermediate U lives in a static 64 MiB
+ * device array that the library holds on every GPU it runs on until the
+ * process ends, so the LDS kernels' grids are limited to NN <= 11 in an FP64
+ * context and NN <= 13 in an FP32 one (larger: DEX_ESIZE); FP64 grids without
+ * the LDS kernels (NN >= 23) use the six-pass kernels, which need no scratch.
+ * Columns go one at a time (no batched launches) and no kernel timing is
+ * recorded.  Calls that overlap on one GPU share the array and race: order them
+ * on one stream or serialise them.  U needs no clearing (each column's first
+ * pass writes all of it). */
+int dex_matvec_noalloc(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
+                       int64_t checksize, int32_t single, int32_t symmetry);
+
 /* MYDOTSUB(BRAS,KETS,SIZE,NN,MM,OUTDOTS), doublyexcited.F90:47-59:
  * OUT(NN,MM) = BRAS^H KETS, column-major.  Host or device pointers. */
 int dex_dots(dex_ctx *ctx, const double *bras, const double *kets, int64_t size,
