@@ -287,45 +287,53 @@ def run_scan(w: dict, args) -> int:
     return 0
 
 
+# iterations of the solo request whose every launch is timed for the roofline
+SOLO_ITERATIONS = 5
+
 # the served request stream (the metric's configuration)
-SERVED = dict(seed=1005, rate=0.5, cap=240e9, slots=4, tol=1e-7, maxit=100,
+SERVED = dict(seed=1005, rate=0.5, cap=240e9, slots=4, tol=1e-7, maxit=40, window=300.0,
               desc="served request stream: Poisson arrivals (seed 1005); NN uniform over 11..21 "
                    "(even snapped up), delta U[0.35, 0.6], theta U[0.2, 0.5], Re sigma from "
                    "{-0.78, -0.69, -0.60}, Im sigma -0.002, block from {4, 8, 16}, tol 1e-7; "
                    "admission by device footprint under a 240 GB cap, up to 4 solves in flight")
 
 
-def _matvec_fraction_by_grid(results) -> dict:
-    """Per grid: device time of the matvec passes per column (event-timed
-    launches) against the bound max(F/peak, B/BW) of one column, with
-    F = (24 NN + 14) NN^6 flops and B = 32 NN^6 bytes (complex128 columns, FP64
-    peak) or 16 NN^6 (complex64 columns, FP32 peak)."""
-    acc = {}
-    for r in results:
-        if not r.ok or not r.kernel_stats:
-            continue
-        a = acc.setdefault(r.nn, [0.0, 0.0, 0.0])
-        for k in ("pass1", "pass2", "symadd"):
-            ms, n, by, fl = r.kernel_stats[k]
-            a[0] += ms
-        ms1, n1, by1, fl1 = r.kernel_stats["pass1"]
-        a[1] += by1  # pass-1 bytes = (X + U entry bytes) x D x columns
-        a[2] += fl1  # 12 NN D per column
+def matvec_fraction_by_grid(grids, delta=0.5, theta=0.3, reps=5) -> dict:
+    """Per grid of the stream: the exchange-symmetric matvec on one column
+    (the solver's form), FP64 on complex128 and FP32 on complex64 columns,
+    each pass timed with HIP events (every launch) over ``reps`` calls, against
+    the bound max(F / peak, B / BW) of one column with F = (24 NN + 14) NN^6
+    flops and B = 32 NN^6 bytes (FP64, 78.6 TFLOP/s) or 16 NN^6 (complex64,
+    157.3 TFLOP/s FP32)."""
+    import torch
+
+    import doublyexcited_amd as dex
+
     out = {}
-    for nn, (ms, by1, fl1) in sorted(acc.items()):
+    for nn in sorted(set(grids)):
+        n = (nn - 1) // 2
         d = float(nn) ** 6
-        cols = fl1 / (12.0 * nn * d) if fl1 > 0 else 0.0
-        if cols <= 0 or ms <= 0:
-            continue
-        per_col_s = ms * 1e-3 / cols
-        entry = by1 / (cols * d)  # 32 (FP64) or 16 (complex64 in and out) bytes per entry
-        fp32 = entry < 24.0
-        f = (24.0 * nn + 14.0) * d
-        b = (16.0 if fp32 else 32.0) * d
-        peak = (157.3 if fp32 else FP64_PEAK_TFS) * 1e12
-        bound = max(f / peak, b / (HBM_PEAK_GBS * 1e9))
-        out[str(nn)] = {"us_per_column": per_col_s * 1e6, "bound_us": bound * 1e6, "frac": bound / per_col_s,
-                        "precision": "fp32" if fp32 else "fp64", "columns_timed": round(cols, 1)}
+        row = {}
+        for prec, dtype, peak, bpe in (("fp64", torch.complex128, FP64_PEAK_TFS, 32.0),
+                                       ("fp32", torch.complex64, 157.3, 16.0)):
+            with dex.HeliumGrid(n, delta, theta, precision=prec) as g:
+                x = torch.randn(g.m, g.m, dtype=dtype, device="cuda")
+                x = (0.5 * (x + x.t())).contiguous().view(-1)
+                y = torch.empty_like(x)
+                g.matvec_device(x, y, 1, single=prec == "fp32", symmetry=1)  # warm
+                g.reset_kernel_stats()
+                g.set_kernel_timing(1)
+                for _ in range(reps):
+                    g.matvec_device(x, y, 1, single=prec == "fp32", symmetry=1)
+                st = g.kernel_stats()
+                g.set_kernel_timing(False)
+                del x, y
+            ms = sum(st[k][0] for k in ("pass1", "symadd", "generic"))
+            per = ms * 1e-3 / reps
+            bound = max((24.0 * nn + 14.0) * d / (peak * 1e12), bpe * d / (HBM_PEAK_GBS * 1e9))
+            row[prec] = {"us_per_column": per * 1e6, "bound_us": bound * 1e6, "frac": bound / per,
+                         "pass1_us": st["pass1"][0] / reps * 1e3, "symadd_us": st["symadd"][0] / reps * 1e3}
+        out[str(nn)] = row
     return out
 
 
@@ -459,10 +467,10 @@ def run_served(args) -> int:
                               maxit=args.maxit or sv["maxit"], tol=sv["tol"],
                               plan_precision="fp64" if args.same_plan else None)
         log(f"timed: {len(reqs)} {form} requests, rate {sv['rate']}/s, {slots} slots")
-        srv = Server(sv["cap"], slots, timing=TIMING_SAMPLE, progress=10)
+        srv = Server(sv["cap"], slots, progress=10)
         torch.cuda.synchronize()
         t0 = time.time()
-        res = srv.run(reqs, realtime=True, on_result=lambda r, q: log(
+        res = srv.run(reqs, realtime=True, window=args.window or sv["window"], on_result=lambda r, q: log(
             f"  {form} request {r.rid}: NN={r.nn} block {r.nev} ns {q.ns} inner {q.inner_iter}/{q.inner_block} "
             f"guard {q.guard}: {r.converged_pairs}/{r.nev} pairs, {r.iterations} iterations, "
             f"latency {r.latency:.1f}s service {r.service:.1f}s" + (f" ERROR {r.error}" if not r.ok else "")))
@@ -473,8 +481,8 @@ def run_served(args) -> int:
         res, wall = out[forms[0]][1], out[forms[0]][2]
         print(json.dumps({"form": forms[0], "wall_s": wall, "converged_pairs": sum(r.converged_pairs for r in res),
                           "latency_s": [r.latency for r in res], "iterations": [r.iterations for r in res],
-                          "roofline": _roofline_from_results(res, TIMING_SAMPLE),
-                          "matvec_roofline_by_grid": _matvec_fraction_by_grid(res)}))
+                          "requests": [{"nn": r.nn, "nev": r.nev, "iterations": r.iterations,
+                                        "pairs": r.converged_pairs, "service_s": r.service} for r in res]}))
         return 0
     res64 = out["fp64"][1]
     wall64 = out["fp64"][2]
@@ -484,7 +492,9 @@ def run_served(args) -> int:
         "fp64": {"value": pairs64 / wall64, "unit": "eigenpairs/s", "wall_s": wall64, "converged_pairs": pairs64,
                  "requested_pairs": sum(r.nev for r in out["fp64"][0]),
                  "latency_s": {"p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99))},
-                 "failed_requests": sum(not r.ok for r in res64),
+                 "served_requests": sum(r.ok for r in res64),
+                 "turned_away": sum((not r.ok) and "window" in (r.error or "") for r in res64),
+                 "failed_requests": sum((not r.ok) and "window" not in (r.error or "") for r in res64),
                  "iterations": [r.iterations for r in res64]},
     }
     head = "fp64"
@@ -497,7 +507,9 @@ def run_served(args) -> int:
         summary["mixed"] = {
             "value": pairsmx / wallmx, "unit": "eigenpairs/s", "wall_s": wallmx, "converged_pairs": pairsmx,
             "latency_s": {"p50": float(np.percentile(latmx, 50)), "p99": float(np.percentile(latmx, 99))},
-            "failed_requests": sum(not r.ok for r in resmx),
+            "served_requests": sum(r.ok for r in resmx),
+            "turned_away": sum((not r.ok) and "window" in (r.error or "") for r in resmx),
+            "failed_requests": sum((not r.ok) and "window" not in (r.error or "") for r in resmx),
             "fp64_restarts_mean": float(np.mean([r.fp64_restarts for r in resmx if r.ok])),
             "iterations": [r.iterations for r in resmx],
             "max_abs_dE_vs_fp64": float(max(de)) if de else None,
@@ -524,7 +536,8 @@ def run_served(args) -> int:
                 "He Hamiltonian built per request",
         "config": {"workload": "he-served-stream", "desc": SERVED["desc"], "form": head,
                    "rate_per_s": SERVED["rate"], "requests": args.steps,
-                   "iteration_budget_per_request": args.maxit or SERVED["maxit"], "parallelism": "1 GPU"},
+                   "iteration_budget_per_request": args.maxit or SERVED["maxit"],
+                   "admission_window_s": args.window or SERVED["window"], "parallelism": "1 GPU"},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "max": float(max(lat))} if lat else None,
         "converged_pairs": pairs,
@@ -532,14 +545,24 @@ def run_served(args) -> int:
         "requests": [{"rid": q.rid, "nn": 2 * q.n + 1, "nev": q.nev, "ns": q.ns, "inner": q.inner_iter,
                       "inner_block": q.inner_block, "guard": q.guard} for q in reqs],
     }
-    result["roofline"] = _roofline_from_results(res, TIMING_SAMPLE)
+    # the roofline: the stream's first request on its largest grid served alone in the headline form for a
+    # few iterations, every launch event-timed (concurrent solves would share the device inside the events)
+    big = max(range(len(reqs)), key=lambda i: (2 * reqs[i].n + 1, reqs[i].nev, -i))
+    solo_req = poisson_stream(sv["rate"], big + 1, sv["seed"], precision=head, plan=True, cap=sv["cap"],
+                              maxit=SOLO_ITERATIONS, tol=sv["tol"],
+                              plan_precision="fp64" if args.same_plan else None)[big]
+    solo_req.arrival = 0.0
+    log(f"roofline pass: request {big} (NN={2 * solo_req.n + 1}, block {solo_req.nev}) alone, "
+        f"{SOLO_ITERATIONS} iterations, every launch timed")
+    solo = Server(sv["cap"], 1, timing=1).run([solo_req], realtime=False)[0]
+    result["roofline"] = _roofline_from_results([solo], 1)
     rf = result["roofline"]
+    rf["workload"] = (f"request {big} of the stream (NN={solo.nn}, block {solo.nev}, {head} form) served alone, "
+                      f"{solo.iterations} iterations")
     if rf.get("kernel"):
         rf["traffic"] = traffic_from_profile("served", rf["kernel"], rf["alg_bytes_per_launch"])
         rf["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
-    result["matvec_roofline_by_grid"] = {"fp64": _matvec_fraction_by_grid(res64)}
-    if "mixed" in out:
-        result["matvec_roofline_by_grid"]["mixed"] = _matvec_fraction_by_grid(out["mixed"][1])
+    result["matvec_roofline_by_grid"] = matvec_fraction_by_grid([2 * q.n + 1 for q in reqs])
     if not args.no_cpu_baseline:
         try:
             result["cpu_baseline"] = served_cpu_baseline(res64)
@@ -561,6 +584,8 @@ def main() -> int:
     ap.add_argument("--forms", default="fp64,mixed", help="served stream: the forms to serve")
     ap.add_argument("--only-forms", action="store_true",
                     help="served stream: serve only --forms (no FP64 reference; a probe, not a bench line)")
+    ap.add_argument("--window", type=float, default=0.0,
+                    help="served stream: seconds during which requests are admitted (0: the workload's, 300)")
     ap.add_argument("--same-plan", action="store_true",
                     help="served stream: both forms use the FP64 form's solver settings (NS, inner solve)")
     ap.add_argument("--maxit", type=int, default=0,

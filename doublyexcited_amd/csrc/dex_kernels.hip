@@ -640,6 +640,42 @@ __device__ __forceinline__ void st_col(void *p, int fmt, int64_t r, double2 v) {
     reinterpret_cast<double2 *>(p)[r] = v;
 }
 
+// Two adjacent rows r, r + 1 of a complex128 (fmt 0) or complex64 (fmt 1)
+// column in FP64; `two` false reads only row r (the last row of an odd-length
+// column).  A complex64 pair is one 16-byte load, a complex128 pair two.
+__device__ __forceinline__ void ld_col2(const void *p, int fmt, int64_t r, bool two, double2 &v0, double2 &v1) {
+  if (fmt != 0) {
+    const float2 *q = reinterpret_cast<const float2 *>(p) + r;
+    if (two) {
+      const float4 w = *reinterpret_cast<const float4 *>(q);
+      v0 = make_double2(double(w.x), double(w.y));
+      v1 = make_double2(double(w.z), double(w.w));
+    } else {
+      const float2 w = *q;
+      v0 = make_double2(double(w.x), double(w.y));
+      v1 = make_double2(0.0, 0.0);
+    }
+    return;
+  }
+  const double2 *q = reinterpret_cast<const double2 *>(p) + r;
+  v0 = q[0];
+  v1 = two ? q[1] : make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ void st_col2(void *p, int fmt, int64_t r, bool two, double2 v0, double2 v1) {
+  if (fmt != 0) {
+    float2 *q = reinterpret_cast<float2 *>(p) + r;
+    if (two)
+      *reinterpret_cast<float4 *>(q) = make_float4(float(v0.x), float(v0.y), float(v1.x), float(v1.y));
+    else
+      *q = make_float2(float(v0.x), float(v0.y));
+    return;
+  }
+  double2 *q = reinterpret_cast<double2 *>(p) + r;
+  q[0] = v0;
+  if (two) q[1] = v1;
+}
+
 __device__ __forceinline__ double2 col_value(const ColDesc &c, int64_t r) {
   const double2 va = ld_col(c.a, c.fmt, r);
   if (c.b == nullptr) return va;
@@ -671,8 +707,9 @@ typedef double dex_v4d __attribute__((ext_vector_type(4)));
 // workgroup with 16-row tiles: 32-row tiles measured 5-6% faster there at
 // NN=21 (8 x 4, 16 x 4, 12 x 10) and 5-7% slower at 28 x 4 and 32 x 16.
 constexpr int gram_mfma_rows(int mt, int nt, int tall = 16) {
+  const int cap = tall >= 64 ? 64 * 1024 : 48 * 1024;  // tiles of up to 64 rows may take 64 KB of LDS
   int r = tall;
-  while (r > 4 && r * (16 * mt + 16 * nt + 1) * 16 > 48 * 1024) r /= 2;
+  while (r > 4 && r * (16 * mt + 16 * nt + 1) * 16 > cap) r /= 2;
   return r;
 }
 
@@ -684,9 +721,12 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
   constexpr int LD = 16 * MT + 16 * NT + 1;
   constexpr int ROWS = gram_mfma_rows(MT, NT, TALL);
   constexpr int QMAX = (MT * NT + 3) / 4;
-  // loads per thread per tile: every thread issues all of them before any
-  // wait, and the next tile's while the matrix cores work on this one
-  constexpr int PMAX = (ROWS * (16 * MT + 16 * NT) + kGramThreads - 1) / kGramThreads;
+  // loads per thread per tile, each an adjacent row pair of one column (16-byte
+  // accesses for complex64 and complex128 alike): every thread issues all of
+  // them before any wait, and the next tile's while the matrix cores work on
+  // this one
+  constexpr int HALF = ROWS / 2;
+  constexpr int PMAX = (HALF * (16 * MT + 16 * NT) + kGramThreads - 1) / kGramThreads;
   __shared__ double2 tile[ROWS * LD];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -709,7 +749,7 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
     accr[q] = dex_v4d{0.0, 0.0, 0.0, 0.0};
     acci[q] = dex_v4d{0.0, 0.0, 0.0, 0.0};
   }
-  // Element p of this thread is (row e % ROWS, column e / ROWS) of every
+  // Element p of this thread is (row pair e % HALF, column e / HALF) of every
   // tile, e = tid + 256 p: its column pointers are fixed for the whole loop.
   const void *pa[PMAX];
   const void *pb[PMAX];
@@ -720,8 +760,8 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
     pa[p] = nullptr;
     pb[p] = nullptr;
     pf[p] = 0;
-    if (e < ROWS * nload) {
-      const ColDesc &dc = descs[e / ROWS];
+    if (e < HALF * nload) {
+      const ColDesc &dc = descs[e / HALF];
       pa[p] = dc.a;
       pb[p] = dc.b;
       pf[p] = dc.fmt;
@@ -729,16 +769,16 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
   }
   const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
-  double2 va[PMAX], vb[PMAX];
+  double2 va[PMAX][2], vb[PMAX][2];
   auto fetch = [&](int64_t rb) {
 #pragma unroll
     for (int p = 0; p < PMAX; ++p) {
-      const int64_t row = rb + (tid + kGramThreads * p) % ROWS;
-      va[p] = make_double2(0.0, 0.0);
-      vb[p] = make_double2(0.0, 0.0);
+      const int64_t row = rb + 2 * ((tid + kGramThreads * p) % HALF);
+      va[p][0] = va[p][1] = vb[p][0] = vb[p][1] = make_double2(0.0, 0.0);
       if (pa[p] != nullptr && row < r1) {
-        va[p] = ld_col(pa[p], pf[p], row);
-        if (pb[p] != nullptr) vb[p] = ld_col(pb[p], pf[p], row);
+        const bool two = row + 1 < r1;
+        ld_col2(pa[p], pf[p], row, two, va[p][0], va[p][1]);
+        if (pb[p] != nullptr) ld_col2(pb[p], pf[p], row, two, vb[p][0], vb[p][1]);
       }
     }
   };
@@ -747,17 +787,20 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
     for (int p = 0; p < PMAX; ++p) {
       if (pa[p] == nullptr) continue;  // beyond the loads, or supplied through a bslot
       const int e = tid + kGramThreads * p;
-      const int r = e % ROWS, c = e / ROWS;
+      const int r = 2 * (e % HALF), c = e / HALF;
       const int slot = c < kl ? c : 16 * MT + (c - kl);
-      if (pb[p] == nullptr) {
-        tile[r * LD + slot] = va[p];
-      } else {
-        const ColDesc &dc = descs[c];
-        double2 o;
-        o.x = va[p].x - (dc.sig_re * vb[p].x - dc.sig_im * vb[p].y);
-        o.y = va[p].y - (dc.sig_re * vb[p].y + dc.sig_im * vb[p].x);
-        tile[r * LD + slot] = o;
-        if (dc.bslot >= 0) tile[r * LD + dc.bslot] = vb[p];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (pb[p] == nullptr) {
+          tile[(r + h) * LD + slot] = va[p][h];
+        } else {
+          const ColDesc &dc = descs[c];
+          double2 o;
+          o.x = va[p][h].x - (dc.sig_re * vb[p][h].x - dc.sig_im * vb[p][h].y);
+          o.y = va[p][h].y - (dc.sig_re * vb[p][h].y + dc.sig_im * vb[p][h].x);
+          tile[(r + h) * LD + slot] = o;
+          if (dc.bslot >= 0) tile[(r + h) * LD + dc.bslot] = vb[p][h];
+        }
       }
     }
   };
@@ -835,8 +878,10 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NP = kCombGroups * JPT;
   // input elements per thread per tile: all issued before any wait, the next
-  // tile's while this one is combined
-  constexpr int PMAX = kCombRows * kCombMaxIn / kCombThreads;
+  // tile's while this one is combined; each an adjacent row pair of one
+  // column (16-byte accesses for complex64 and complex128 alike)
+  constexpr int HALF = kCombRows / 2;
+  constexpr int PMAX = HALF * kCombMaxIn / kCombThreads;
   double2 *sc = reinterpret_cast<double2 *>(smem);  // [nin][NP]
   double2 *tin = sc + size_t(nin) * NP;              // [nin][kCombRows]
   const int tid = threadIdx.x;
@@ -846,7 +891,7 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
     const int i = e / NP, j = e % NP;
     sc[e] = j < nout ? cm[i + size_t(nin) * j] : make_double2(0.0, 0.0);
   }
-  // element p of this thread: row e % kCombRows of input column e / kCombRows
+  // element p of this thread: row pair e % HALF of input column e / HALF
   const void *pa[PMAX];
   const void *pb[PMAX];
   int pf[PMAX];
@@ -856,21 +901,21 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
     pa[p] = nullptr;
     pb[p] = nullptr;
     pf[p] = 0;
-    if (e < kCombRows * nin) {
-      const ColDesc &dc = descs[e / kCombRows];
+    if (e < HALF * nin) {
+      const ColDesc &dc = descs[e / HALF];
       pa[p] = dc.a;
       pb[p] = dc.b;
       pf[p] = dc.fmt;
     }
   }
-  double2 va[PMAX], vb[PMAX];
+  double2 va[PMAX][2], vb[PMAX][2];
   auto fetch = [&](int64_t rb) {
 #pragma unroll
     for (int p = 0; p < PMAX; ++p) {
-      const int64_t r = rb + (tid + kCombThreads * p) % kCombRows;
+      const int64_t r = rb + 2 * ((tid + kCombThreads * p) % HALF);
       if (pa[p] != nullptr && r < d) {
-        va[p] = ld_col(pa[p], pf[p], r);
-        if (pb[p] != nullptr) vb[p] = ld_col(pb[p], pf[p], r);
+        ld_col2(pa[p], pf[p], r, r + 1 < d, va[p][0], va[p][1]);
+        if (pb[p] != nullptr) ld_col2(pb[p], pf[p], r, r + 1 < d, vb[p][0], vb[p][1]);
       }
     }
   };
@@ -884,15 +929,18 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
     for (int p = 0; p < PMAX; ++p) {
       if (pa[p] == nullptr) continue;
       const int e = tid + kCombThreads * p;
-      const int r = e % kCombRows, c = e / kCombRows;
-      if (r >= nr) continue;
-      double2 v = va[p];
-      if (pb[p] != nullptr) {
-        const ColDesc &dc = descs[c];
-        v.x -= dc.sig_re * vb[p].x - dc.sig_im * vb[p].y;
-        v.y -= dc.sig_re * vb[p].y + dc.sig_im * vb[p].x;
+      const int r = 2 * (e % HALF), c = e / HALF;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (r + h >= nr) continue;
+        double2 v = va[p][h];
+        if (pb[p] != nullptr) {
+          const ColDesc &dc = descs[c];
+          v.x -= dc.sig_re * vb[p][h].x - dc.sig_im * vb[p][h].y;
+          v.y -= dc.sig_re * vb[p][h].y + dc.sig_im * vb[p][h].x;
+        }
+        tin[c * kCombRows + r + h] = v;
       }
-      tin[c * kCombRows + r] = v;
     }
     __syncthreads();
     if (tb + gridDim.x < ntile) fetch((tb + gridDim.x) * kCombRows);
@@ -936,23 +984,30 @@ __global__ __launch_bounds__(kCombThreads) void k_combine(const ColDesc *__restr
 
 // The combine for narrow outputs (nout <= 16; also 33..48, where the LDS
 // kernel's staging would exceed 64 KB per workgroup), without LDS: a wavefront
-// owns 64 * RPL consecutive rows (RPL per lane, 64 apart) and every output
-// column of them, so the coefficients C(i, j) are wavefront-uniform (scalar
-// loads, no LDS traffic) and each input element is read once, by one lane, in
-// coalesced 1 KB row segments.  JN, the output width the registers hold, is
-// the smallest of 4, 8, 12, 16, 48 that covers nout: spare accumulators cost
-// occupancy.  RPL = 2 with UNR inputs unrolled keeps 2 UNR loads in flight per
-// wavefront (about 4.4 TB/s at 10 outputs against 3.3 with one row per lane).
-// All inputs of a row are read before any output of it is written, so outputs
-// may alias inputs.
+// owns 64 * RPL consecutive rows and every output column of them, so the
+// coefficients C(i, j) are wavefront-uniform (scalar loads, no LDS traffic)
+// and each input element is read once, by one lane.  With RPL = 2 a lane owns
+// two adjacent rows, so every load and store is 16 bytes wide whatever the
+// storage (one complex64 pair, or two complex128 halves of a 32-byte run) and a
+// wavefront reads 2 KB (complex128) or 1 KB (complex64) contiguous per column:
+// with one complex64 row per lane the loads were 8 bytes wide and the kernel
+// moved half the bytes per instruction.  With RPL = 1 a lane owns one row.
+// JN, the output width the registers hold, is the smallest of 4, 8, 12, 16, 48
+// that covers nout: spare accumulators cost occupancy.  UNR inputs are
+// unrolled.  All inputs of a row are read before any output of it is written,
+// so outputs may alias inputs.  Columns start 256-byte aligned and row pairs
+// start at even rows, so the 16-byte accesses are aligned.
 template <int JN, int RPL, int UNR>
 __global__ __launch_bounds__(256) void k_combine_w(const ColDesc *__restrict__ descs, int nin, int nout,
                                                    const double2 *__restrict__ cm,
                                                    const double2 *__restrict__ beta, int64_t d) {
+  static_assert(RPL == 1 || RPL == 2, "one row or an adjacent row pair per lane");
   const int lane = threadIdx.x & 63;
   const int64_t wave = int64_t(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t stride = int64_t(gridDim.x) * 4 * 64 * RPL;
   for (int64_t rb = wave * 64 * RPL; rb < d; rb += stride) {
+    const int64_t r0 = rb + RPL * lane;  // this lane's first row
+    const bool ok0 = r0 < d, ok1 = RPL == 2 && r0 + 1 < d;
     double ax[RPL][JN], ay[RPL][JN];
 #pragma unroll
     for (int k = 0; k < RPL; ++k)
@@ -963,15 +1018,24 @@ __global__ __launch_bounds__(256) void k_combine_w(const ColDesc *__restrict__ d
       const ColDesc dc = descs[i];
       double2 v[RPL];
 #pragma unroll
-      for (int k = 0; k < RPL; ++k) {
-        const int64_t r = rb + 64 * k + lane;
-        v[k] = make_double2(0.0, 0.0);
-        if (r < d) {
-          v[k] = ld_col(dc.a, dc.fmt, r);
+      for (int k = 0; k < RPL; ++k) v[k] = make_double2(0.0, 0.0);
+      if (ok0) {
+        if constexpr (RPL == 2) {
+          ld_col2(dc.a, dc.fmt, r0, ok1, v[0], v[RPL - 1]);
           if (dc.b != nullptr) {
-            const double2 vb = ld_col(dc.b, dc.fmt, r);
-            v[k].x -= dc.sig_re * vb.x - dc.sig_im * vb.y;
-            v[k].y -= dc.sig_re * vb.y + dc.sig_im * vb.x;
+            double2 b0, b1;
+            ld_col2(dc.b, dc.fmt, r0, ok1, b0, b1);
+            v[0].x -= dc.sig_re * b0.x - dc.sig_im * b0.y;
+            v[0].y -= dc.sig_re * b0.y + dc.sig_im * b0.x;
+            v[RPL - 1].x -= dc.sig_re * b1.x - dc.sig_im * b1.y;
+            v[RPL - 1].y -= dc.sig_re * b1.y + dc.sig_im * b1.x;
+          }
+        } else {
+          v[0] = ld_col(dc.a, dc.fmt, r0);
+          if (dc.b != nullptr) {
+            const double2 vb = ld_col(dc.b, dc.fmt, r0);
+            v[0].x -= dc.sig_re * vb.x - dc.sig_im * vb.y;
+            v[0].y -= dc.sig_re * vb.y + dc.sig_im * vb.x;
           }
         }
       }
@@ -989,24 +1053,32 @@ __global__ __launch_bounds__(256) void k_combine_w(const ColDesc *__restrict__ d
         }
       }
     }
+    if (!ok0) continue;
 #pragma unroll
-    for (int k = 0; k < RPL; ++k) {
-      const int64_t r = rb + 64 * k + lane;
-      if (r >= d) continue;
+    for (int q = 0; q < JN; ++q) {
+      if (q < nout) {
+        const ColDesc &od = descs[nin + q];
+        void *o = const_cast<Dbl2 *>(od.a);
+        double2 out[RPL];
 #pragma unroll
-      for (int q = 0; q < JN; ++q) {
-        if (q < nout) {
-          const ColDesc &od = descs[nin + q];
-          void *o = const_cast<Dbl2 *>(od.a);
-          double ox = ax[k][q], oy = ay[k][q];
-          if (beta != nullptr) {
-            const double2 bq = beta[q];
-            const double2 ov = ld_col(o, od.fmt, r);
-            ox += bq.x * ov.x - bq.y * ov.y;
-            oy += bq.x * ov.y + bq.y * ov.x;
+        for (int k = 0; k < RPL; ++k) out[k] = make_double2(ax[k][q], ay[k][q]);
+        if (beta != nullptr) {
+          const double2 bq = beta[q];
+          double2 ov[RPL];
+          if constexpr (RPL == 2)
+            ld_col2(o, od.fmt, r0, ok1, ov[0], ov[RPL - 1]);
+          else
+            ov[0] = ld_col(o, od.fmt, r0);
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) {
+            out[k].x += bq.x * ov[k].x - bq.y * ov[k].y;
+            out[k].y += bq.x * ov[k].y + bq.y * ov[k].x;
           }
-          st_col(o, od.fmt, r, make_double2(ox, oy));
         }
+        if constexpr (RPL == 2)
+          st_col2(o, od.fmt, r0, ok1, out[0], out[RPL - 1]);
+        else
+          st_col(o, od.fmt, r0, out[0]);
       }
     }
   }
@@ -1324,17 +1396,29 @@ int gram_grid() {
   return grid;
 }
 
-// 32-row tiles for the narrow shapes: on unless DEX_GRAM_ROWS=16 (read once).
-bool gram_tall() {
-  static const bool tall = [] {
+// Tile height choice (read once): DEX_GRAM_ROWS=16 keeps 16-row tiles
+// everywhere, 64 takes up-to-64-row tiles (64 KB of LDS) for every shape;
+// otherwise 32-row tiles for single-tile Grams and 16 elsewhere.
+int gram_rows_choice() {
+  static const int rows = [] {
     const char *v = std::getenv("DEX_GRAM_ROWS");
-    return !(v && std::atoi(v) == 16);
+    return v ? std::atoi(v) : 32;
   }();
-  return tall;
+  return rows;
 }
 
 template <int MA>
 void gram_grids_nb() {
+  if (gram_rows_choice() == 64) {
+    (void)gram_grid<MA, 1, 64>();
+    (void)gram_grid<MA, 2, 64>();
+    (void)gram_grid<MA, 3, 64>();
+    (void)gram_grid<MA, 4, 64>();
+    (void)gram_grid<MA, 5, 64>();
+    (void)gram_grid<MA, 6, 64>();
+    (void)gram_grid<MA, 7, 64>();
+    (void)gram_grid<MA, 8, 64>();
+  }
   (void)gram_grid<MA, 1, 16>();
   (void)gram_grid<MA, 2, 16>();
   (void)gram_grid<MA, 3, 16>();
@@ -1360,8 +1444,10 @@ int gram_launch_t(const ColDesc *descs, int kl, int kr, int share, int64_t d, do
 template <int MA, int NB>
 int gram_launch(const ColDesc *descs, int kl, int kr, int share, int64_t d, double2 *partial,
                 hipStream_t s) {
+  const int rows = gram_rows_choice();
+  if (rows == 64) return gram_launch_t<MA, NB, 64>(descs, kl, kr, share, d, partial, s);
   if constexpr (MA == 1 && NB == 1) {
-    if (gram_tall()) return gram_launch_t<MA, NB, 32>(descs, kl, kr, share, d, partial, s);
+    if (rows != 16) return gram_launch_t<MA, NB, 32>(descs, kl, kr, share, d, partial, s);
   }
   return gram_launch_t<MA, NB, 16>(descs, kl, kr, share, d, partial, s);
 }
@@ -1441,7 +1527,7 @@ hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c
       return dim3(unsigned((waves + 3) / 4 < 2048 ? (waves + 3) / 4 : 2048));
     };
     if (nout <= 4)
-      hipLaunchKernelGGL((k_combine_w<4, 1, 4>), grid(1), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
+      hipLaunchKernelGGL((k_combine_w<4, 2, 8>), grid(2), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
     else if (nout <= 8)
       hipLaunchKernelGGL((k_combine_w<8, 2, 8>), grid(2), dim3(256), 0, s, descs, nin, nout, cc, bb, d);
     else if (nout <= 12)

@@ -208,10 +208,13 @@ class Server:
             return Result(req.rid, False, None, None, 0, 0, done - req.arrival, done - t_admit,
                           error=repr(exc))
 
-    def run(self, requests: list[Request], realtime: bool = True, on_result=None) -> list[Result]:
+    def run(self, requests: list[Request], realtime: bool = True, on_result=None,
+            window: float | None = None) -> list[Result]:
         """Serves the requests (arrival times relative to the start); returns
         results in request order.  ``on_result(result, request)`` is called as
-        each request completes."""
+        each request completes.  With ``window`` (seconds) the service admits
+        nothing after that time: requests still waiting then are turned away
+        (an error result each) and the solves in flight finish."""
         pending = sorted(requests)
         results: dict[int, Result] = {}
         threads: list[threading.Thread] = []
@@ -220,6 +223,15 @@ class Server:
         i = 0
         while i < len(pending) or waiting:
             now = time.time() - t0
+            if window is not None and now > window:
+                closed = [w[2] for w in waiting] + list(pending[i:])
+                with self._cv:
+                    for req in closed:
+                        results[req.rid] = Result(req.rid, False, None, None, 0, 0, now - min(now, req.arrival),
+                                                  0.0, error=f"not admitted within the {window:.0f} s window",
+                                                  nn=2 * req.n + 1, nev=req.nev)
+                waiting, i = [], len(pending)
+                break
             while i < len(pending) and (pending[i].arrival <= now or not realtime):
                 heapq.heappush(waiting, (pending[i].arrival, pending[i].rid, pending[i]))
                 i += 1

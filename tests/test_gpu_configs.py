@@ -9,8 +9,9 @@ host with the C restatement of HAMMULT (oracle/hammult.c) at <= tol; the
 numpy GPLMR restatement itself is too slow at this size (about 13 s per
 iteration on one host core) and is compared with the GPU solver step by step
 at NN=5 and 7 in tests/test_gpu_gplmr.py.  Independence from the start: a
-solve from another seed returns the same eigenvalues (|dE| <= 1e-9 au), and the
-mixed form returns them to 1e-7 au.
+solve from another seed returns the same eigenvalues to the tolerance (|dE| <=
+1e-8 au: a Rayleigh quotient of this non-normal operator is accurate to about
+its residual), and the mixed form returns them to 1e-7 au.
 """
 import numpy as np
 import pytest
@@ -66,7 +67,9 @@ def test_config1_block8_pairs_by_c_oracle_residuals(dex):
     print("case: config 1 another start (seed 1002)", flush=True)
     r2, _ = _solve(dex, 1002)
     assert r2.converged
-    assert match(r.e, r2.e) <= 1e-9 and match(r2.e, r.e) <= 1e-9
+    # converged to residual 1e-8; a Hermitian Rayleigh quotient of this non-normal operator is
+    # accurate to about the residual, so two starts agree to the tolerance
+    assert match(r.e, r2.e) <= TOL and match(r2.e, r.e) <= TOL
 
     print("case: config 1 mixed form", flush=True)
     r3, x3 = _solve(dex, 1001, "mixed")

@@ -45,3 +45,12 @@ def test_oversized_request_is_rejected_without_stopping_the_service():
     big = Request(arrival=0.0, rid=0, n=10, delta=0.4, theta=0.3, sigma=-0.7 + 0j, nev=16, tol=1e-7)
     out = s.run([big], realtime=False)
     assert not out[0].ok and "admission" in out[0].error
+
+
+def test_requests_after_the_admission_window_are_turned_away():
+    s = Server(hbm_cap_bytes=240e9, slots=2)
+    reqs = [Request(arrival=0.5 * i, rid=i, n=5, delta=0.5, theta=0.3, sigma=-0.78 + 0j, nev=4, tol=1e-7)
+            for i in range(3)]
+    out = s.run(reqs, realtime=True, window=0.0)
+    assert all((not r.ok) and "window" in r.error for r in out)
+    assert [r.rid for r in out] == [0, 1, 2]
