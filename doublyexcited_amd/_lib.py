@@ -30,7 +30,7 @@ KERNEL_CLASSES = ("pass1", "pass2", "symadd", "gram", "combine", "coldots", "gen
 
 # every symbol include/dex.h declares
 EXPORTS = [
-    "dex_params_default", "dex_ctx_create", "dex_ctx_destroy", "dex_ctx_info",
+    "dex_params_default", "dex_ctx_create", "dex_ctx_destroy", "dex_ctx_info", "dex_ctx_stop",
     "dex_get_kematrix", "dex_get_tinv", "dex_matvec", "dex_matvec_device", "dex_matvec_sym_device",
     "dex_matvec_noalloc",
     "dex_dots",
@@ -124,6 +124,7 @@ def load() -> C.CDLL:
         "dex_status_string": (C.c_char_p, [C.c_int]),
         "dex_last_error": (C.c_char_p, []),
         "dex_set_kernel_timing": (C.c_int, [P, C.c_int32]),
+        "dex_ctx_stop": (C.c_int, [P, C.c_int32]),
         "dex_get_kernel_timing": (C.c_int, [P, dp, dp]),
         "dex_get_kernel_stats": (C.c_int, [P, C.c_int32, dp, C.POINTER(C.c_int64), dp, dp]),
         "dex_get_kernel_totals": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int64), dp, dp]),

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -99,6 +100,7 @@ struct dex_ctx {
   size_t partial_bytes = 0;
   double *d_dots = nullptr;
   int timing = 0;                 // 0 off, 1 every launch, n > 1 a random 1 in n of them
+  std::atomic<int> stop{0};       // dex_ctx_stop: solves return at their next iteration boundary
   uint64_t sample_state = 0x9e3779b97f4a7c15ull;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double t_pass1 = -1.0, t_pass2 = -1.0;
@@ -514,6 +516,44 @@ int coldots(dex_ctx *c, const std::vector<ColDesc> &a, const std::vector<ColDesc
   return 0;
 }
 
+// The residual step per column j: X_j <- s_j X_j, AX_j <- s_j AX_j,
+// W_j = AX_j - E_j X_j (scaled), and rn_j = ||W_j||, in one pass over X and AX.
+int residual_cols(dex_ctx *c, const std::vector<Dbl2 *> &x, const std::vector<Dbl2 *> &ax,
+                  const std::vector<Dbl2 *> &w, const std::vector<double> &sc, const std::vector<cd> &ev,
+                  std::vector<double> &rn) {
+  const int nev = int(x.size());
+  if (nev < 1 || int(ax.size()) != nev || int(w.size()) < nev || nev > 64)
+    return fail(DEX_EINVAL, "residual step: bad block widths");
+  std::vector<ColDesc> ds;
+  for (Dbl2 *p : x) ds.push_back(plain(p));
+  for (Dbl2 *p : ax) ds.push_back(plain(p));
+  for (int j = 0; j < nev; ++j) ds.push_back(plain(w[j]));
+  // descriptors | scalings | E in one image, one copy
+  const size_t bd = (ds.size() * sizeof(ColDesc) + 255) / 256 * 256;
+  const size_t bs = (sc.size() * sizeof(double) + 255) / 256 * 256;
+  std::vector<unsigned char> img(bd + bs + ev.size() * sizeof(cd));
+  std::memcpy(img.data(), ds.data(), ds.size() * sizeof(ColDesc));
+  std::memcpy(img.data() + bd, sc.data(), sc.size() * sizeof(double));
+  std::memcpy(img.data() + bd + bs, ev.data(), ev.size() * sizeof(cd));
+  DEX_HIP(stage_upload(c, c->d_args, img.data(), img.size()));
+  {
+    double bytes = 0.0;  // read X and AX, write X, AX and W
+    for (int j = 0; j < nev; ++j)
+      bytes += 2.0 * (is32(x[j]) ? 8.0 : 16.0) + 2.0 * (is32(ax[j]) ? 8.0 : 16.0) + (is32(w[j]) ? 8.0 : 16.0);
+    KTimer kt(c, DEX_K_COLDOTS, double(c->d) * bytes, 12.0 * double(c->d) * nev);
+    DEX_HIP(dex::launch_resid(reinterpret_cast<const ColDesc *>(c->d_args), nev,
+                              reinterpret_cast<const double *>(c->d_args + bd),
+                              reinterpret_cast<const Dbl2 *>(c->d_args + bd + bs), c->d,
+                              reinterpret_cast<double *>(c->d_partial), c->d_dots, c->stream));
+  }
+  std::vector<double> h(nev);
+  DEX_HIP(hipMemcpyAsync(h.data(), c->d_dots, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DEX_HIP(hipStreamSynchronize(c->stream));
+  rn.resize(nev);
+  for (int j = 0; j < nev; ++j) rn[j] = std::sqrt(h[j]);
+  return 0;
+}
+
 std::vector<ColDesc> descs_of(const Block &b) {
   std::vector<ColDesc> v;
   for (Dbl2 *p : b.cols) v.push_back(plain(p));
@@ -691,6 +731,12 @@ int dex_get_tinv(const dex_ctx *c, double *tinv) {
   if (!c || !tinv) return fail(DEX_EINVAL, "null argument");
   std::memcpy(tinv, c->tinv_host.data(), c->tinv_host.size() * sizeof(double));
   return DEX_OK;
+}
+
+int dex_ctx_stop(dex_ctx *c, int32_t stop) {
+  if (!c) return fail(DEX_EINVAL, "null context");
+  c->stop.store(stop != 0 ? 1 : 0, std::memory_order_relaxed);
+  return 0;
 }
 
 int dex_set_kernel_timing(dex_ctx *c, int32_t enable) {
@@ -1539,36 +1585,16 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     if (int st = coldots(c, descs_of(X), descs_of(AX), aa, ab, bb)) return st;
     S.t_gram += now_s() - t0;
     t0 = now_s();
-    std::vector<cd> cm(size_t(2 * nev) * 3 * nev, 0.0);
-    const int ld = 2 * nev;
+    // per column: X_j / n, AX_j / n and W_j = AX_j/n - E_j X_j/n with ||W_j||, one pass
+    std::vector<double> sc(nev);
+    std::vector<cd> ev(nev);
     for (int j = 0; j < nev; ++j) {
-      const double n = std::sqrt(aa[j]);
-      const double dinv = 1.0 / n;
+      sc[j] = 1.0 / std::sqrt(aa[j]);
       e[j] = ab[j] / aa[j];
-      cm[j + size_t(ld) * j] = dinv;                      // X_j / n
-      cm[nev + j + size_t(ld) * (nev + j)] = dinv;        // AX_j / n
-      cm[nev + j + size_t(ld) * (2 * nev + j)] = dinv;    // W_j = AX_j/n - X_j E_j / n
-      cm[j + size_t(ld) * (2 * nev + j)] = -e[j] * dinv;
+      ev[j] = e[j];
     }
-    std::vector<ColDesc> in = descs_of(X);
-    for (Dbl2 *p : AX.cols) in.push_back(plain(p));
-    std::vector<Dbl2 *> out = X.cols;
-    out.insert(out.end(), AX.cols.begin(), AX.cols.end());
-    out.insert(out.end(), W.cols.begin(), W.cols.end());
-    if (3 * nev <= 48) {
-      if (int st = combine(c, in, out, cm)) return st;
-    } else {
-      // more outputs than one launch takes: W first, then X and AX in place
-      const std::vector<cd> cw(cm.begin() + size_t(ld) * 2 * nev, cm.end());
-      const std::vector<cd> cx(cm.begin(), cm.begin() + size_t(ld) * 2 * nev);
-      if (int st = combine(c, in, W.cols, cw)) return st;
-      if (int st = combine(c, in, std::vector<Dbl2 *>(out.begin(), out.begin() + 2 * nev), cx)) return st;
-    }
+    if (int st = residual_cols(c, X.cols, AX.cols, W.cols, sc, ev, resn)) return st;
     S.t_update += now_s() - t0;
-    t0 = now_s();
-    if (int st = coldots(c, descs_of(W), {}, aa, ab, bb)) return st;
-    S.t_gram += now_s() - t0;
-    for (int j = 0; j < nev; ++j) resn[j] = std::sqrt(aa[j]);
     return 0;
   };
   // First E and W: X is already unit, the scaling is 1.
@@ -1595,7 +1621,7 @@ int gplmr_run(dex_ctx *c, const std::vector<Dbl2 *> &xcols, int nconv, cd *e_out
     ++refreshes;
     return 0;
   };
-  while (iter <= maxit && (maxres > tol || !fresh)) {
+  while (iter <= maxit && (maxres > tol || !fresh) && c->stop.load(std::memory_order_relaxed) == 0) {
     if (lowp && maxres <= tol && !fresh) {
       // converged on the recurrence: check against an FP64 AX
       if (int st = refresh_ax()) return st;
@@ -1934,7 +1960,7 @@ extern "C" int dex_gplmr_eig_device(dex_ctx *c, void *x_dev, double *e, int32_t 
   // restarts from its X (each with a fresh FP64 AX and no P block).
   st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol, inner_iter, outmodulus, o, res, mixed, store);
   if (mixed && st == DEX_ENOTCONVERGED) {
-    for (int r = 0; r < o.fp64_restarts; ++r) {
+    for (int r = 0; r < o.fp64_restarts && c->stop.load(std::memory_order_relaxed) == 0; ++r) {
       dex_result sub{};
       st = gplmr_run(c, xcols, nev, ev.data(), sigma, ns, maxit, tol, inner_iter, outmodulus, o,
                      res ? &sub : nullptr, true, store);

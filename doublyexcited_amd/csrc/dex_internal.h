@@ -113,6 +113,13 @@ size_t coldots_partial_elems(int k);
 hipError_t launch_coldots(const ColDesc *descs, int k, int64_t d, double *partial, double *out,
                           hipStream_t s);
 
+// The GPLMR residual step per column j: X_j <- s_j X_j, AX_j <- s_j AX_j,
+// W_j = AX_j - E_j X_j (scaled columns); out[j] = ||W_j||^2.  descs: X, AX, W
+// (nev each); sc (nev reals) and ev (nev complex E_j) device arrays; `partial`
+// holds coldots_partial_elems(nev) doubles.
+hipError_t launch_resid(const ColDesc *descs, int nev, const double *sc, const Dbl2 *ev, int64_t d,
+                        double *partial, double *out, hipStream_t s);
+
 // Philox4x32-10 complex Gaussian fill: element i of column c gets
 // (g0 + i g1)/sqrt(2) with g from Box-Muller on the counter (i, c, stream, 0).
 hipError_t launch_fill_gaussian(Dbl2 *x, int64_t n, uint64_t seed, uint32_t col, uint32_t stream_id,

@@ -1127,6 +1127,49 @@ __global__ __launch_bounds__(kDotThreads) void k_coldots(const ColDesc *__restri
   }
 }
 
+// The residual step of GPLMR (gplmr.F90:324-341) for column j = blockIdx.y
+// after the scaling s_j = 1 / ||X_j|| and the Rayleigh quotient E_j are known:
+// X_j <- s_j X_j, AX_j <- s_j AX_j, W_j = AX_j - E_j X_j (scaled), one pass over
+// X and AX (row pairs, 16-byte accesses), with each workgroup's partial sum of
+// |W_j|^2 (in FP64, before W is rounded to its storage) in
+// partial[blockIdx.x * nev + j].  descs: X (nev), AX (nev), W (nev).
+__global__ __launch_bounds__(256) void k_resid(const ColDesc *__restrict__ descs, int nev,
+                                               const double *__restrict__ sc, const double2 *__restrict__ ev,
+                                               int64_t d, int64_t rows_per_block, double *__restrict__ partial) {
+  const int j = blockIdx.y;
+  const ColDesc cx = descs[j], cax = descs[nev + j], cw = descs[2 * nev + j];
+  const double s = sc[j];
+  const double2 f = ev[j];
+  const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
+  double acc = 0.0;
+  for (int64_t r = r0 + 2 * int64_t(threadIdx.x); r < r1; r += 2 * int64_t(blockDim.x)) {
+    const bool two = r + 1 < r1;
+    double2 x[2], a[2], w[2];
+    ld_col2(cx.a, cx.fmt, r, two, x[0], x[1]);
+    ld_col2(cax.a, cax.fmt, r, two, a[0], a[1]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      x[h].x *= s;
+      x[h].y *= s;
+      a[h].x *= s;
+      a[h].y *= s;
+      w[h].x = a[h].x - (f.x * x[h].x - f.y * x[h].y);
+      w[h].y = a[h].y - (f.x * x[h].y + f.y * x[h].x);
+    }
+    st_col2(const_cast<Dbl2 *>(cx.a), cx.fmt, r, two, x[0], x[1]);
+    st_col2(const_cast<Dbl2 *>(cax.a), cax.fmt, r, two, a[0], a[1]);
+    st_col2(const_cast<Dbl2 *>(cw.a), cw.fmt, r, two, w[0], w[1]);
+    acc += w[0].x * w[0].x + w[0].y * w[0].y;
+    if (two) acc += w[1].x * w[1].x + w[1].y * w[1].y;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[size_t(blockIdx.x) * nev + j] = red[0] + red[1] + red[2] + red[3];
+}
+
 __global__ __launch_bounds__(256) void k_reduce_real(const double *__restrict__ partial, int nparts,
                                                      int nelem, double *__restrict__ out) {
   const int lane = threadIdx.x & 63;
@@ -1515,9 +1558,28 @@ hipError_t combine_launch(const ColDesc *descs, int nin, int nout, const Dbl2 *c
 }
 }  // namespace
 
+namespace {
+// 17..32 outputs without LDS (k_combine_w with 32 accumulators, one row per
+// lane) instead of the LDS kernel: on with DEX_COMB_W32=1 (read once).
+bool comb_w32_on() {
+  static const bool on = [] {
+    const char *v = std::getenv("DEX_COMB_W32");
+    return v && std::atoi(v) == 1;
+  }();
+  return on;
+}
+}  // namespace
+
 hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
                           int64_t d, hipStream_t s) {
   if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombMaxOut) return hipErrorInvalidValue;
+  if (nout > 16 && nout <= 32 && comb_w32_on()) {
+    const int64_t waves = (d + 63) / 64;
+    const dim3 g(unsigned((waves + 3) / 4 < 2048 ? (waves + 3) / 4 : 2048));
+    hipLaunchKernelGGL((k_combine_w<32, 1, 4>), g, dim3(256), 0, s, descs, nin, nout,
+                       reinterpret_cast<const double2 *>(c), reinterpret_cast<const double2 *>(beta), d);
+    return hipGetLastError();
+  }
   if (nout <= 16 || nout > kCombGroups * 4) {
     const double2 *cc = reinterpret_cast<const double2 *>(c);
     const double2 *bb = reinterpret_cast<const double2 *>(beta);
@@ -1547,6 +1609,19 @@ hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c
 }
 
 size_t coldots_partial_elems(int k) { return size_t(kDotBlocks) * k * 4; }
+
+hipError_t launch_resid(const ColDesc *descs, int nev, const double *sc, const Dbl2 *ev, int64_t d,
+                        double *partial, double *out, hipStream_t s) {
+  if (nev < 1 || nev > kDotMaxCols) return hipErrorInvalidValue;
+  // kDotBlocks * 4 partial sums per column at most: the coldots partial buffer holds them
+  const int nb = kDotBlocks;
+  int64_t rpb = (d + nb - 1) / nb;
+  rpb = (rpb + 1) / 2 * 2;  // row pairs start at even rows
+  hipLaunchKernelGGL(k_resid, dim3(unsigned(nb), unsigned(nev)), dim3(256), 0, s, descs, nev, sc,
+                     reinterpret_cast<const double2 *>(ev), d, rpb, partial);
+  hipLaunchKernelGGL(k_reduce_real, dim3((nev + 3) / 4), dim3(256), 0, s, partial, nb, nev, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_coldots(const ColDesc *descs, int k, int64_t d, double *partial, double *out,
                           hipStream_t s) {

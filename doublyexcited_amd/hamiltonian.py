@@ -187,6 +187,12 @@ class HeliumGrid:
     def sync(self):
         check(self._lib.dex_sync(self._ctx))
 
+    def stop(self, stop: bool = True):
+        """Asks a solve running on this grid's context (from any thread) to
+        return at its next iteration boundary; later solves return at their
+        first until ``stop(False)``."""
+        check(self._lib.dex_ctx_stop(self._ctx, int(bool(stop))))
+
     def set_kernel_timing(self, enable: bool | int = True):
         """False/0 off, True/1 every launch, n > 1 a random ~1 in n launches."""
         check(self._lib.dex_set_kernel_timing(self._ctx, int(enable)))

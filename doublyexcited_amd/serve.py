@@ -58,6 +58,7 @@ class Result:
     nev: int = 0
     kernel_stats: dict | None = None    # sampled event timings per kernel class
     kernel_totals: dict | None = None   # every launch: (count, algorithmic bytes, flops) per class
+    stopped: bool = False               # ended unconverged by the service's horizon
 
 
 def snap_nn(nn: int) -> int:
@@ -174,56 +175,91 @@ class Server:
         self._in_use = 0.0
         self._free_slots = slots
         self._cv = threading.Condition(self._lock)
+        self._active: dict = {}     # rid -> grid of each solve in flight
+        self._stopping = False      # the horizon has passed: solves are asked to stop
 
-    def _solve(self, req: Request, t_admit: float, t0: float) -> Result:
-        """Runs one request; t_admit and the returned times are relative to t0."""
+    def _register(self, rid: int, g) -> None:
+        with self._cv:
+            self._active[rid] = g
+            if self._stopping:
+                g.stop()
+
+    def _stop_all(self) -> None:
+        with self._cv:
+            self._stopping = True
+            for g in self._active.values():
+                g.stop()
+
+    def _execute(self, req: Request):
+        """The request's solve on the device: (solve result, FP64 residuals of
+        the returned pairs, sampled kernel statistics, kernel totals).  The
+        grid is registered for ``_stop_all`` while its solve runs."""
         import torch
 
         import doublyexcited_amd as dex
 
-        try:
-            torch.cuda.set_device(self.device)
-            with dex.HeliumGrid(req.n, req.delta, req.theta, precision=req.precision,
-                                device=self.device) as g:
-                x = torch.empty(g.d * req.nev, dtype=torch.complex128, device=f"cuda:{self.device}")
-                g.fill_gaussian_device(x, req.nev, seed=req.seed, symmetry=1)
-                if self.timing:
-                    g.set_kernel_timing(self.timing)
+        torch.cuda.set_device(self.device)
+        with dex.HeliumGrid(req.n, req.delta, req.theta, precision=req.precision, device=self.device) as g:
+            x = torch.empty(g.d * req.nev, dtype=torch.complex128, device=f"cuda:{self.device}")
+            g.fill_gaussian_device(x, req.nev, seed=req.seed, symmetry=1)
+            if self.timing:
+                g.set_kernel_timing(self.timing)
+            self._register(req.rid, g)
+            try:
                 r = g.gplmr_eig_device(x, req.sigma, req.nev, ns=req.ns, maxit=req.maxit, tol=req.tol,
                                        symmetry=1, fp32_tol=max(req.tol, 1e-5), inner_iter=req.inner_iter,
                                        inner_block=req.inner_block, guard=req.guard,
                                        verbose=2 if self.progress else False, outmodulus=self.progress)
-                stats = g.kernel_stats() if self.timing else None
-                totals = g.kernel_totals()
-                g.set_kernel_timing(False)
-                res = g.residuals_device(x, r.e)
-                del x
+            finally:
+                with self._cv:
+                    self._active.pop(req.rid, None)
+            stats = g.kernel_stats() if self.timing else None
+            totals = g.kernel_totals()
+            g.set_kernel_timing(False)
+            res = g.residuals_device(x, r.e)
+            del x
+        return r, res, stats, totals
+
+    def _solve(self, req: Request, t_admit: float, t0: float) -> Result:
+        """Runs one request; t_admit and the returned times are relative to t0."""
+        try:
+            r, res, stats, totals = self._execute(req)
             done = time.time() - t0
             conv = int(np.sum(res <= req.tol))
+            # ended early, unconverged, after the horizon's stop request
+            stopped = self._stopping and not r.converged and r.iterations < req.maxit
             return Result(req.rid, True, r.e, res, conv, r.iterations, done - req.arrival,
                           done - t_admit, matvecs=r.matvecs, fp64_restarts=r.fp64_restarts_used,
-                          nn=2 * req.n + 1, nev=req.nev, kernel_stats=stats, kernel_totals=totals)
+                          nn=2 * req.n + 1, nev=req.nev, kernel_stats=stats, kernel_totals=totals,
+                          stopped=bool(stopped))
         except Exception as exc:  # isolate the failure to this request
             done = time.time() - t0
             return Result(req.rid, False, None, None, 0, 0, done - req.arrival, done - t_admit,
                           error=repr(exc))
 
     def run(self, requests: list[Request], realtime: bool = True, on_result=None,
-            window: float | None = None) -> list[Result]:
+            window: float | None = None, horizon: float | None = None) -> list[Result]:
         """Serves the requests (arrival times relative to the start); returns
         results in request order.  ``on_result(result, request)`` is called as
         each request completes.  With ``window`` (seconds) the service admits
         nothing after that time: requests still waiting then are turned away
-        (an error result each) and the solves in flight finish."""
+        (an error result each) and the solves in flight finish.  With
+        ``horizon`` (seconds) the service ends then: admission closes as with
+        a window, and every solve still in flight is asked to return at its
+        next iteration boundary (its result holds the pairs converged by then
+        and ``stopped`` is set)."""
         pending = sorted(requests)
         results: dict[int, Result] = {}
         threads: list[threading.Thread] = []
+        self._stopping = False
         t0 = time.time()
         waiting: list = []  # heap of (arrival, rid, req) that arrived but are not admitted
+        close = min(v for v in (window, horizon) if v is not None) if (window, horizon) != (None, None) else None
         i = 0
         while i < len(pending) or waiting:
             now = time.time() - t0
-            if window is not None and now > window:
+            if close is not None and now > close:
+                window = close
                 closed = [w[2] for w in waiting] + list(pending[i:])
                 with self._cv:
                     for req in closed:
@@ -272,6 +308,11 @@ class Server:
                     if i < len(pending) and realtime:
                         timeout = max(0.0, min(timeout, pending[i].arrival - (time.time() - t0)))
                     self._cv.wait(timeout=timeout if timeout > 0 else 0.001)
+        if horizon is not None:
+            with self._cv:
+                while any(th.is_alive() for th in threads) and time.time() - t0 < horizon:
+                    self._cv.wait(timeout=max(0.001, min(0.5, horizon - (time.time() - t0))))
+            self._stop_all()
         for th in threads:
             th.join()
         return [results[r.rid] for r in requests]

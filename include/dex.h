@@ -79,6 +79,15 @@ int dex_ctx_create(const dex_params *p, dex_ctx **out);
 int dex_ctx_destroy(dex_ctx *ctx);
 int dex_ctx_info(const dex_ctx *ctx, dex_info *info);
 
+/* stop = 1 asks a solve running on this context (dex_gplmr_eig_device and
+ * its restarts) to return at its next iteration boundary, with its current
+ * X and E and DEX_ENOTCONVERGED unless it had converged; later solves on the
+ * context return at their first boundary until stop = 0 clears the request.
+ * Safe to call from another thread while the solve runs.  No counterpart in
+ * the reference (one run per process, ended by MAXIT): it is for servers that
+ * end work at a deadline. */
+int dex_ctx_stop(dex_ctx *ctx, int32_t stop);
+
 /* Copies of the host-side tables (for checking): T(NN,NN) real before the
  * 1/SCALE^2 factor, Tinv(-NN:NN)^3 real (the reference's HAM%TINV). */
 int dex_get_kematrix(const dex_ctx *ctx, double *t /* NN*NN */);
@@ -325,7 +334,7 @@ enum {
   DEX_K_SYMADD = 2,  /* exchange transpose-add pass with the potential */
   DEX_K_GRAM = 3,    /* block Gram (with its partial-sum reduction) */
   DEX_K_COMBINE = 4, /* block linear combination */
-  DEX_K_COLDOTS = 5, /* per-column dots and norms */
+  DEX_K_COLDOTS = 5, /* per-column dots and norms, and the residual step (scaled X, AX and W) */
   DEX_K_GENERIC = 6, /* generic-grid matvec */
   DEX_KCLASSES = 7
 };

@@ -59,3 +59,36 @@ def test_admission_footprint_is_the_library_count(dex, n):
             for nev, ns, inner, ib, guard in ((4, 1, 0, 0, 0), (8, 4, 8, 0, 2), (16, 1, 4, 4, 0)):
                 assert g.footprint(nev, ns, inner, ib, guard) == \
                     footprint_bytes(n, nev, prec, ns, inner, ib, guard)
+
+
+def test_stop_ends_a_solve_at_an_iteration_boundary(dex):
+    """dex_ctx_stop: set before a solve, the solve returns at its first
+    boundary with the start block's Rayleigh quotients; set from another
+    thread during a solve, it returns early; cleared, solves run to MAXIT."""
+    import threading
+    import time
+
+    print("case: stop flag", flush=True)
+    sigma = -0.78 - 0.002j
+    for prec in ("fp64", "mixed"):
+        with dex.HeliumGrid(3, 0.5, 0.3, precision=prec) as g:
+            x = torch.empty(g.d * 4, dtype=torch.complex128, device="cuda")
+            g.fill_gaussian_device(x, 4, seed=21, symmetry=1)
+            g.stop()
+            r = g.gplmr_eig_device(x, sigma, 4, ns=1, maxit=50, tol=1e-14, symmetry=1)
+            assert r.iterations == 0 and not r.converged
+            assert np.all(np.isfinite(r.e))
+            g.stop(False)
+            r = g.gplmr_eig_device(x, sigma, 4, ns=1, maxit=3, tol=1e-14, symmetry=1)
+            assert r.iterations == 3
+            g.stop(False)
+    # from another thread, mid-solve (NN=11): it returns, converged or stopped, long before MAXIT
+    with dex.HeliumGrid(5, 0.5, 0.3) as g:
+        x = torch.empty(g.d * 4, dtype=torch.complex128, device="cuda")
+        g.fill_gaussian_device(x, 4, seed=22, symmetry=1)
+        t = threading.Timer(0.3, g.stop)
+        t.start()
+        t0 = time.time()
+        r = g.gplmr_eig_device(x, sigma, 4, ns=1, maxit=100000, tol=1e-12, symmetry=1)
+        t.join()
+        assert 0 < r.iterations < 100000 and time.time() - t0 < 60.0

@@ -54,3 +54,47 @@ def test_requests_after_the_admission_window_are_turned_away():
     out = s.run(reqs, realtime=True, window=0.0)
     assert all((not r.ok) and "window" in r.error for r in out)
     assert [r.rid for r in out] == [0, 1, 2]
+
+
+def test_horizon_stops_solves_in_flight_and_turns_away_the_rest():
+    """At the horizon the server asks every solve in flight to stop (the
+    library returns at the next iteration boundary) and turns away requests
+    still waiting.  The device solve is replaced by one that runs until its
+    grid is stopped, so only the server's bookkeeping is exercised."""
+    import threading
+    import time
+    from types import SimpleNamespace
+
+    class FakeGrid:
+        def __init__(self):
+            self.ev = threading.Event()
+
+        def stop(self, stop=True):
+            self.ev.set()
+
+    class FakeServer(Server):
+        def _execute(self, req):
+            g = FakeGrid()
+            self._register(req.rid, g)
+            try:
+                assert g.ev.wait(10.0), "never stopped"
+            finally:
+                with self._cv:
+                    self._active.pop(req.rid, None)
+            r = SimpleNamespace(converged=False, iterations=3, e=np.zeros(req.nev, complex), matvecs=12,
+                                fp64_restarts_used=0)
+            return r, np.array([1e-9] + [1.0] * (req.nev - 1)), None, None
+
+    s = FakeServer(hbm_cap_bytes=240e9, slots=1)
+    reqs = [Request(arrival=0.0, rid=i, n=5, delta=0.5, theta=0.3, sigma=-0.78 + 0j, nev=4, tol=1e-7, maxit=40)
+            for i in range(3)]
+    t0 = time.time()
+    out = s.run(reqs, realtime=True, horizon=0.3)
+    assert time.time() - t0 < 5.0
+    assert out[0].ok and out[0].stopped and out[0].converged_pairs == 1 and out[0].iterations == 3
+    # one slot: the other two were still waiting at the horizon
+    assert all((not r.ok) and "window" in r.error for r in out[1:])
+    # a request registered after the horizon is stopped at once
+    g = FakeGrid()
+    s._register(99, g)
+    assert g.ev.is_set()
