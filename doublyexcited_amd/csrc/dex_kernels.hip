@@ -767,8 +767,13 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
       pf[p] = dc.fmt;
     }
   }
-  const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
-  const int64_t r1 = r0 + rows_per_block < d ? r0 + rows_per_block : d;
+  // rows_per_block > 0: this workgroup's rows are one contiguous range; 0: the
+  // workgroups take tiles round robin (tile t to workgroup t % gridDim.x), so
+  // the grid sweeps every column together through a narrow window of rows
+  const bool strided = rows_per_block == 0;
+  const int64_t r0 = strided ? int64_t(blockIdx.x) * ROWS : int64_t(blockIdx.x) * rows_per_block;
+  const int64_t r1 = strided ? d : (r0 + rows_per_block < d ? r0 + rows_per_block : d);
+  const int64_t rstep = strided ? int64_t(gridDim.x) * ROWS : ROWS;
   double2 va[PMAX][2], vb[PMAX][2];
   auto fetch = [&](int64_t rb) {
 #pragma unroll
@@ -805,10 +810,10 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
     }
   };
   if (r0 < r1) fetch(r0);
-  for (int64_t rb = r0; rb < r1; rb += ROWS) {
+  for (int64_t rb = r0; rb < r1; rb += rstep) {
     store();
     __syncthreads();
-    if (rb + ROWS < r1) fetch(rb + ROWS);
+    if (rb + rstep < r1) fetch(rb + rstep);
 #pragma unroll 2
     for (int k0 = 0; k0 < ROWS; k0 += 4) {
       const double2 *trow = tile + (k0 + lk) * LD;
@@ -1473,12 +1478,23 @@ void gram_grids_nb() {
   if constexpr (MA == 1) (void)gram_grid<MA, 1, 32>();
 }
 
+// Round-robin tiles (DEX_GRAM_STRIDE=1, read once) instead of one contiguous
+// row range per workgroup.
+bool gram_stride_on() {
+  static const bool on = [] {
+    const char *v = std::getenv("DEX_GRAM_STRIDE");
+    return v && std::atoi(v) == 1;
+  }();
+  return on;
+}
+
 template <int MA, int NB, int TALL>
 int gram_launch_t(const ColDesc *descs, int kl, int kr, int share, int64_t d, double2 *partial,
                   hipStream_t s) {
   const int grid = gram_grid<MA, NB, TALL>();
   int64_t rpb = (d + grid - 1) / grid;
   rpb = (rpb + 63) / 64 * 64;  // a whole number of tiles for every tile height
+  if (gram_stride_on()) rpb = 0;
   hipLaunchKernelGGL((k_gram_mfma<MA, NB, TALL>), dim3(grid), dim3(kGramThreads), 0, s, descs, kl, kr, share,
                      d, rpb, partial);
   return grid;
@@ -1558,28 +1574,9 @@ hipError_t combine_launch(const ColDesc *descs, int nin, int nout, const Dbl2 *c
 }
 }  // namespace
 
-namespace {
-// 17..32 outputs without LDS (k_combine_w with 32 accumulators, one row per
-// lane) instead of the LDS kernel: on with DEX_COMB_W32=1 (read once).
-bool comb_w32_on() {
-  static const bool on = [] {
-    const char *v = std::getenv("DEX_COMB_W32");
-    return v && std::atoi(v) == 1;
-  }();
-  return on;
-}
-}  // namespace
-
 hipError_t launch_combine(const ColDesc *descs, int nin, int nout, const Dbl2 *c, const Dbl2 *beta,
                           int64_t d, hipStream_t s) {
   if (nin < 1 || nin > kCombMaxIn || nout < 1 || nout > kCombMaxOut) return hipErrorInvalidValue;
-  if (nout > 16 && nout <= 32 && comb_w32_on()) {
-    const int64_t waves = (d + 63) / 64;
-    const dim3 g(unsigned((waves + 3) / 4 < 2048 ? (waves + 3) / 4 : 2048));
-    hipLaunchKernelGGL((k_combine_w<32, 1, 4>), g, dim3(256), 0, s, descs, nin, nout,
-                       reinterpret_cast<const double2 *>(c), reinterpret_cast<const double2 *>(beta), d);
-    return hipGetLastError();
-  }
   if (nout <= 16 || nout > kCombGroups * 4) {
     const double2 *cc = reinterpret_cast<const double2 *>(c);
     const double2 *bb = reinterpret_cast<const double2 *>(beta);

@@ -165,9 +165,15 @@ class Server:
     every ``progress`` iterations."""
 
     def __init__(self, hbm_cap_bytes: float = 240e9, slots: int = 4, device: int = 0, timing: int = 0,
-                 progress: int = 0):
+                 progress: int = 0, policy: str = "fifo"):
+        if policy not in ("fifo", "smallest"):
+            raise ValueError("policy must be 'fifo' or 'smallest'")
         self.cap = float(hbm_cap_bytes)
         self.slots = slots
+        # "fifo": requests are admitted in arrival order (one that does not fit yet holds the
+        # queue); "smallest": the waiting request with the smallest footprint that fits
+        # goes first (a large request waits while smaller ones keep the device busy)
+        self.policy = policy
         self.device = device
         self.timing = int(timing)
         self.progress = int(progress)  # > 0: a progress line to stderr every this many iterations
@@ -274,17 +280,34 @@ class Server:
             admitted = False
             with self._cv:
                 if waiting and self._free_slots > 0:
-                    _, _, req = waiting[0]
+                    pick = 0
+                    if self.policy == "smallest":
+                        # the smallest waiting request that fits now (else the smallest, which waits)
+                        needs = [footprint_bytes(w[2].n, w[2].nev, w[2].precision, w[2].ns, w[2].inner_iter,
+                                                 w[2].inner_block, w[2].guard) for w in waiting]
+                        order = sorted(range(len(waiting)), key=lambda k: (needs[k], waiting[k][:2]))
+                        fits = [k for k in order if self._in_use + needs[k] <= self.cap or needs[k] > self.cap]
+                        pick = fits[0] if fits else order[0]
+
+                    def take(k=pick):
+                        if k == 0:
+                            heapq.heappop(waiting)
+                        else:
+                            waiting[k] = waiting[-1]
+                            waiting.pop()
+                            heapq.heapify(waiting)
+
+                    _, _, req = waiting[pick]
                     need = footprint_bytes(req.n, req.nev, req.precision, req.ns, req.inner_iter,
                                            req.inner_block, req.guard)
                     if need > self.cap:
-                        heapq.heappop(waiting)
+                        take()
                         results[req.rid] = Result(req.rid, False, None, None, 0, 0, 0.0, 0.0,
                                                   error=f"admission: {need/1e9:.1f} GB exceeds the "
                                                         f"{self.cap/1e9:.0f} GB cap")
                         admitted = True
                     elif self._in_use + need <= self.cap:
-                        heapq.heappop(waiting)
+                        take()
                         self._in_use += need
                         self._free_slots -= 1
                         t_admit = time.time() - t0

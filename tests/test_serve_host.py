@@ -98,3 +98,26 @@ def test_horizon_stops_solves_in_flight_and_turns_away_the_rest():
     g = FakeGrid()
     s._register(99, g)
     assert g.ev.is_set()
+
+
+def test_smallest_first_policy_admits_by_footprint():
+    """policy="smallest": with one slot and every request waiting, requests run
+    in order of their device footprint; "fifo" keeps arrival order."""
+    from types import SimpleNamespace
+
+    class FakeServer(Server):
+        def _execute(self, req):
+            r = SimpleNamespace(converged=True, iterations=1, e=np.zeros(req.nev, complex), matvecs=1,
+                                fp64_restarts_used=0)
+            return r, np.zeros(req.nev), None, None
+
+    specs = [(10, 16), (5, 4), (7, 8), (5, 16), (6, 4)]
+    reqs = [Request(arrival=0.0, rid=i, n=n, delta=0.5, theta=0.3, sigma=-0.78 + 0j, nev=nev, tol=1e-7)
+            for i, (n, nev) in enumerate(specs)]
+    need = [footprint_bytes(q.n, q.nev, "fp64") for q in reqs]
+    for policy, want in (("smallest", sorted(range(5), key=lambda k: need[k])), ("fifo", list(range(5)))):
+        seen = []
+        out = FakeServer(hbm_cap_bytes=240e9, slots=1, policy=policy).run(
+            reqs, realtime=False, on_result=lambda r, q: seen.append(r.rid))
+        assert all(r.ok for r in out)
+        assert seen == want, (policy, seen)
