@@ -431,15 +431,50 @@ def served_cpu_baseline(res64, budget_s: float = 25.0) -> dict:
     }
 
 
-def traffic_run_served(out_path: str, rid: int, form: str) -> int:
+def roofline_request(steps: int, form: str, same_plan: bool = False):
+    """The request the roofline pass serves alone: the stream's first request
+    on its largest grid (widest block on ties), with the solver settings of
+    ``form`` and SOLO_ITERATIONS iterations.  Returns (index, request)."""
+    from doublyexcited_amd.serve import poisson_stream
+
+    sv = SERVED
+    reqs = poisson_stream(sv["rate"], steps, sv["seed"], precision=form, plan=True, cap=sv["cap"],
+                          maxit=SOLO_ITERATIONS, tol=sv["tol"], plan_precision="fp64" if same_plan else None)
+    big = max(range(len(reqs)), key=lambda i: (2 * reqs[i].n + 1, reqs[i].nev, -i))
+    req = reqs[big]
+    req.arrival = 0.0
+    return big, req
+
+
+def roofline_pass(steps: int, form: str, same_plan: bool = False) -> dict:
+    """The roofline of the dominant kernel class: the roofline request
+    (``roofline_request``) served alone in ``form``, every launch bracketed by
+    HIP events on its context's stream (concurrent solves would share the
+    device inside the events)."""
+    from doublyexcited_amd.serve import Server
+
+    big, req = roofline_request(steps, form, same_plan)
+    log(f"roofline pass: request {big} (NN={2 * req.n + 1}, block {req.nev}, {form}) alone, "
+        f"{SOLO_ITERATIONS} iterations, every launch timed")
+    solo = Server(SERVED["cap"], 1, timing=1).run([req], realtime=False)[0]
+    rf = _roofline_from_results([solo], 1)
+    rf["workload"] = (f"request {big} of the stream (NN={solo.nn}, block {solo.nev}, {form} form) served alone, "
+                      f"{solo.iterations} iterations")
+    if rf.get("kernel"):
+        rf["traffic"] = traffic_from_profile("served", rf["kernel"], rf["alg_bytes_per_launch"])
+        rf["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
+    return rf
+
+
+def traffic_run_served(out_path: str, steps: int, form: str) -> int:
     """The program the HBM-traffic counter passes profile for the served
     stream: a calibration kernel of known size (an elementwise multiply reading
-    and writing 1 GiB), then request ``rid`` of the stream served alone in
-    ``form``; every launch's algorithmic bytes per kernel class (the library's
-    totals) go to ``out_path``."""
+    and writing 1 GiB), then the roofline pass's request (``roofline_request``)
+    served alone in ``form``; every launch's algorithmic bytes per kernel class
+    (the library's totals) go to ``out_path``."""
     import torch
 
-    from doublyexcited_amd.serve import Server, poisson_stream
+    from doublyexcited_amd.serve import Server
 
     torch.cuda.set_device(0)
     n = 64 * 1024 * 1024
@@ -451,10 +486,8 @@ def traffic_run_served(out_path: str, rid: int, form: str) -> int:
     del a, b
     torch.cuda.empty_cache()
     sv = SERVED
-    req = poisson_stream(sv["rate"], rid + 1, sv["seed"], precision=form, plan=True, cap=sv["cap"],
-                         maxit=sv["maxit"], tol=sv["tol"])[rid]
-    req.arrival = 0.0
-    r = Server(sv["cap"], 1, progress=10).run([req], realtime=False)[0]
+    rid, req = roofline_request(steps, form)
+    r = Server(sv["cap"], 1).run([req], realtime=False)[0]
     with open(out_path, "w") as fh:
         json.dump({"request": rid, "form": form, "nn": r.nn, "nev": r.nev, "iterations": r.iterations,
                    "totals": {k: list(v) for k, v in (r.kernel_totals or {}).items()}}, fh)
@@ -593,23 +626,7 @@ def run_served(args) -> int:
         "requests": [{"rid": q.rid, "nn": 2 * q.n + 1, "nev": q.nev, "ns": q.ns, "inner": q.inner_iter,
                       "inner_block": q.inner_block, "guard": q.guard} for q in reqs],
     }
-    # the roofline: the stream's first request on its largest grid served alone in the headline form for a
-    # few iterations, every launch event-timed (concurrent solves would share the device inside the events)
-    big = max(range(len(reqs)), key=lambda i: (2 * reqs[i].n + 1, reqs[i].nev, -i))
-    solo_req = poisson_stream(sv["rate"], big + 1, sv["seed"], precision=head, plan=True, cap=sv["cap"],
-                              maxit=SOLO_ITERATIONS, tol=sv["tol"],
-                              plan_precision="fp64" if args.same_plan else None)[big]
-    solo_req.arrival = 0.0
-    log(f"roofline pass: request {big} (NN={2 * solo_req.n + 1}, block {solo_req.nev}) alone, "
-        f"{SOLO_ITERATIONS} iterations, every launch timed")
-    solo = Server(sv["cap"], 1, timing=1).run([solo_req], realtime=False)[0]
-    result["roofline"] = _roofline_from_results([solo], 1)
-    rf = result["roofline"]
-    rf["workload"] = (f"request {big} of the stream (NN={solo.nn}, block {solo.nev}, {head} form) served alone, "
-                      f"{solo.iterations} iterations")
-    if rf.get("kernel"):
-        rf["traffic"] = traffic_from_profile("served", rf["kernel"], rf["alg_bytes_per_launch"])
-        rf["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
+    result["roofline"] = roofline_pass(args.steps, head, args.same_plan)
     result["matvec_roofline_by_grid"] = matvec_fraction_by_grid([2 * q.n + 1 for q in reqs])
     if not args.no_cpu_baseline:
         try:
@@ -641,10 +658,11 @@ def main() -> int:
     ap.add_argument("--mixed-inner-fp32", action="store_true",
                     help="mixed form: complex64 Arnoldi basis in the inner solve as well")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="served stream: only the roofline pass of the last of --forms (the program the "
+                         "kernel-statistics profile runs); prints its roofline object")
     ap.add_argument("--traffic-run", metavar="STATS_JSON",
                     help="only run the program the HBM-traffic counter passes profile")
-    ap.add_argument("--traffic-request", type=int, default=6,
-                    help="served stream: the request the traffic run serves (default 6: NN=15, block 8)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: independent solves "
@@ -652,7 +670,13 @@ def main() -> int:
         return 2
     if args.workload == "he-served-stream":
         if args.traffic_run:
-            return traffic_run_served(args.traffic_run, args.traffic_request, args.forms.split(",")[-1])
+            return traffic_run_served(args.traffic_run, args.steps, args.forms.split(",")[-1])
+        if args.roofline_only:
+            import torch
+
+            torch.cuda.set_device(0)
+            print(json.dumps(roofline_pass(args.steps, args.forms.split(",")[-1], args.same_plan)))
+            return 0
         return run_served(args)
     import numpy as np
     import torch

@@ -23,6 +23,40 @@ def short(name: str) -> str:
     return n
 
 
+# kernel -> the library's kernel class (the classes of its launch statistics):
+# (class, whether a dispatch of it counts as one launch of the class).  A Gram
+# launch is the MFMA partial kernel plus its partial-sum reduction, a residual
+# step is its kernel plus a reduction, both counted once.
+CLASSES = (
+    ("k_gram_mfma", "gram", True), ("k_reduce_partials", "gram", False),
+    ("k_combine", "combine", True),
+    ("k_pass1", "pass1", True), ("k_symadd", "symadd", True), ("k_pass2", "pass2", True),
+    ("k_coldots", "coldots", True), ("k_resid", "coldots", True), ("k_reduce_real", "coldots", False),
+)
+
+
+def kernel_class(name: str):
+    n = short(name)
+    for prefix, cls, counts in CLASSES:
+        if n.split("<")[0].split()[-1] == prefix or n.split("<")[0].split()[-1].startswith(prefix + "_"):
+            return cls, counts
+    return None, False
+
+
+def class_table(rows) -> list[str]:
+    tot = {}
+    for r in rows:
+        cls, counts = kernel_class(r["Name"])
+        if cls is None:
+            continue
+        t, n = tot.get(cls, (0.0, 0))
+        tot[cls] = (t + float(r["TotalDurationNs"]), n + (int(r["Calls"]) if counts else 0))
+    out = ["| class | launches | total ms | avg us per launch |", "|---|---:|---:|---:|"]
+    for cls, (t, n) in sorted(tot.items(), key=lambda kv: -kv[1][0]):
+        out.append(f"| {cls} | {n} | {t / 1e6:.1f} | {t / n / 1e3 if n else 0.0:.1f} |")
+    return out
+
+
 def summarize(path: str, title: str = "", command: str = "") -> str:
     rows = list(csv.DictReader(open(path, newline="")))
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
@@ -41,6 +75,10 @@ def summarize(path: str, title: str = "", command: str = "") -> str:
             continue
         out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {t / 1e6:.1f} | {float(r['AverageNs']) / 1e3:.1f} | "
                    f"{100.0 * t / tot:.2f} |")
+    out.append("")
+    out.append("Per kernel class (the library's launch-statistics classes; a Gram launch includes its "
+               "partial-sum reduction):\n")
+    out.extend(class_table(rows))
     return "\n".join(out) + "\n"
 
 

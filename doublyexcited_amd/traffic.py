@@ -29,10 +29,19 @@ CLASSES = {
     "symadd": ("k_symadd",),
     "gram": ("k_gram_mfma", "k_reduce_partials"),
     "combine": ("k_combine",),
-    "coldots": ("k_coldots", "k_reduce_real"),
+    "coldots": ("k_coldots", "k_resid", "k_reduce_real"),
     "generic": ("k_axis", "k_potential"),
 }
+# partial-sum reductions that finish a launch of their class (not launches themselves)
+HELPERS = ("k_reduce_partials", "k_reduce_real")
 CAL_BYTES = 1 << 30  # the calibration kernel reads and writes 1 GiB
+
+
+def _base(name: str) -> str:
+    """The kernel's own name: no return type, namespaces, template or
+    parameter list."""
+    n = name.replace("(anonymous namespace)::", "")
+    return n.split("<")[0].split("(")[0].replace("void ", "").strip().split("::")[-1]
 
 
 def _rows(d: Path, counter: str):
@@ -57,7 +66,7 @@ def _per_dispatch(rows):
 
 
 def _cls(name: str):
-    base = name.split("<")[0].replace("void ", "").strip()
+    base = _base(name)
     for c, prefixes in CLASSES.items():
         if any(base.startswith(p) for p in prefixes):
             return c
@@ -67,11 +76,11 @@ def _cls(name: str):
 def analyse(fetch_dir, write_dir, stats_json):
     fetch = _per_dispatch(_rows(fetch_dir, "FETCH_SIZE"))
     write = _per_dispatch(_rows(write_dir, "WRITE_SIZE"))
-    # calibration: the elementwise multiply (the first kernel whose name is not the library's)
+    # calibration: the elementwise multiply, the largest counter value among torch's kernels
+    # (the fill before it writes as much and reads nothing)
     def cal(acc):
-        cands = [(d, v) for (n, d), v in acc.items() if _cls(n) is None and v > 0]
-        cands.sort()
-        return cands[0][1] if cands else None
+        cands = [v for (n, d), v in acc.items() if "at::native" in n and v > 0]
+        return max(cands) if cands else None
     cf, cw = cal(fetch), cal(write)
     f_scale = CAL_BYTES / (cf * 1024.0)
     w_scale = CAL_BYTES / (cw * 1024.0)
@@ -80,10 +89,8 @@ def analyse(fetch_dir, write_dir, stats_json):
     table = []
     ratios = {}
     for c in CLASSES:
-        fv = [v for (n, _), v in fetch.items() if _cls(n) == c and n.split("<")[0].replace("void ", "").strip()
-              .startswith(CLASSES[c][0])]
-        wv = [v for (n, _), v in write.items() if _cls(n) == c and n.split("<")[0].replace("void ", "").strip()
-              .startswith(CLASSES[c][0])]
+        fv = [v for (n, _), v in fetch.items() if _cls(n) == c and not _base(n).startswith(HELPERS)]
+        wv = [v for (n, _), v in write.items() if _cls(n) == c and not _base(n).startswith(HELPERS)]
         if not fv or c not in totals:
             continue
         launches, alg_bytes = totals[c][0], totals[c][1]
