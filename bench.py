@@ -24,6 +24,12 @@ with the pairs converged by then.  Both forms serve the same requests:
          complex64 with FP32 matvecs, every Gram / combination / Cholesky /
          projected pencil in FP64.
 
+Both forms solve each request with the same settings (NS, inner solve, guard
+columns: the FP64 form's plan for the 240 GB cap) and the server reserves the
+FP64 footprint for each in both, so both forms admit the stream alike and
+differ in precision alone; ``--own-plan`` lets the mixed form plan and reserve
+for its own, smaller footprint (wider projections, more solves in flight).
+
 ``value`` is the mixed form's converged eigenpairs (residual recomputed in FP64
 from the returned vector <= tol) per second of its timed region; latency runs
 from arrival to completion over the requests that completed within the
@@ -466,7 +472,7 @@ def roofline_pass(steps: int, form: str, same_plan: bool = False) -> dict:
     return rf
 
 
-def traffic_run_served(out_path: str, steps: int, form: str) -> int:
+def traffic_run_served(out_path: str, steps: int, form: str, same_plan: bool = True) -> int:
     """The program the HBM-traffic counter passes profile for the served
     stream: a calibration kernel of known size (an elementwise multiply reading
     and writing 1 GiB), then the roofline pass's request (``roofline_request``)
@@ -486,7 +492,7 @@ def traffic_run_served(out_path: str, steps: int, form: str) -> int:
     del a, b
     torch.cuda.empty_cache()
     sv = SERVED
-    rid, req = roofline_request(steps, form)
+    rid, req = roofline_request(steps, form, same_plan)
     r = Server(sv["cap"], 1).run([req], realtime=False)[0]
     with open(out_path, "w") as fh:
         json.dump({"request": rid, "form": form, "nn": r.nn, "nev": r.nev, "iterations": r.iterations,
@@ -528,6 +534,9 @@ def run_served(args) -> int:
         reqs = poisson_stream(sv["rate"], args.steps, sv["seed"], precision=form, plan=True, cap=sv["cap"],
                               maxit=args.maxit or sv["maxit"], tol=sv["tol"],
                               plan_precision="fp64" if args.same_plan else None)
+        if args.same_plan:
+            for q in reqs:
+                q.reserve_precision = "fp64"  # admitted as the FP64 form's requests are
         log(f"timed: {len(reqs)} {form} requests, rate {sv['rate']}/s, {slots} slots")
         srv = Server(sv["cap"], slots, progress=10, policy=sv["policy"])
         torch.cuda.synchronize()
@@ -616,7 +625,9 @@ def run_served(args) -> int:
         "config": {"workload": "he-served-stream", "desc": SERVED["desc"], "form": head,
                    "rate_per_s": SERVED["rate"], "requests": args.steps,
                    "iteration_budget_per_request": args.maxit or SERVED["maxit"],
-                   "horizon_s": horizon, "admission": SERVED["policy"], "parallelism": "1 GPU"},
+                   "horizon_s": horizon, "admission": SERVED["policy"],
+                   "solver_settings": "the FP64 form's plan and reservation in both forms" if args.same_plan
+                   else "each form's own plan and footprint", "parallelism": "1 GPU"},
         "latency_s": latency_stats(lat),
         "latency_note": "arrival to completion of the requests that completed within the horizon "
                         "(converged or at their iteration budget); requests stopped at the horizon "
@@ -651,8 +662,9 @@ def main() -> int:
                     help="served stream: serve only --forms (no FP64 reference; a probe, not a bench line)")
     ap.add_argument("--horizon", type=float, default=0.0,
                     help="served stream: seconds each form is served (0: the workload's, 120)")
-    ap.add_argument("--same-plan", action="store_true",
-                    help="served stream: both forms use the FP64 form's solver settings (NS, inner solve)")
+    ap.add_argument("--own-plan", dest="same_plan", action="store_false",
+                    help="served stream: the mixed form plans its own solver settings (NS, inner solve) "
+                         "for its smaller footprint; default: both forms use the FP64 form's settings")
     ap.add_argument("--maxit", type=int, default=0,
                     help="served stream: iteration budget per request (0: the workload's, 40)")
     ap.add_argument("--mixed-inner-fp32", action="store_true",
@@ -670,7 +682,7 @@ def main() -> int:
         return 2
     if args.workload == "he-served-stream":
         if args.traffic_run:
-            return traffic_run_served(args.traffic_run, args.steps, args.forms.split(",")[-1])
+            return traffic_run_served(args.traffic_run, args.steps, args.forms.split(",")[-1], args.same_plan)
         if args.roofline_only:
             import torch
 

@@ -721,23 +721,31 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
   constexpr int LD = 16 * MT + 16 * NT + 1;
   constexpr int ROWS = gram_mfma_rows(MT, NT, TALL);
   constexpr int QMAX = (MT * NT + 3) / 4;
+  // With fewer 16 x 16 output tiles than waves, KS waves share each tile and
+  // split its rows (k steps) between them, their sums added through LDS at the
+  // end: with one wave per tile, a one-tile Gram ran all its MFMAs on one SIMD
+  // of four and every workgroup waited on them
+  constexpr int KS = MT * NT == 1 ? 4 : (MT * NT == 2 ? 2 : 1);
   // loads per thread per tile, each an adjacent row pair of one column (16-byte
   // accesses for complex64 and complex128 alike): every thread issues all of
   // them before any wait, and the next tile's while the matrix cores work on
   // this one
   constexpr int HALF = ROWS / 2;
   constexpr int PMAX = (HALF * (16 * MT + 16 * NT) + kGramThreads - 1) / kGramThreads;
-  __shared__ double2 tile[ROWS * LD];
+  // the row tile, reused after the loop for the k-split sums (4 KB per wave)
+  constexpr int TILE2 = ROWS * LD > MT * NT * (KS - 1) * 256 ? ROWS * LD : MT * NT * (KS - 1) * 256;
+  __shared__ double2 tile[TILE2];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
+  const int ks = KS > 1 ? wave % KS : 0;  // this wave's share of the k steps
   const int nload = kl + (kr - share);
   const int mt_valid = (kl + 15) / 16, nt_valid = (kr + 15) / 16;
   int tm[QMAX], tn[QMAX];
   bool tv[QMAX];
 #pragma unroll
   for (int q = 0; q < QMAX; ++q) {
-    const int t = wave + 4 * q;
+    const int t = KS > 1 ? wave / KS : wave + 4 * q;
     tm[q] = t % MT;
     tn[q] = t / MT;
     tv[q] = t < MT * NT && tm[q] < mt_valid && tn[q] < nt_valid &&
@@ -774,24 +782,23 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
   const int64_t r0 = strided ? int64_t(blockIdx.x) * ROWS : int64_t(blockIdx.x) * rows_per_block;
   const int64_t r1 = strided ? d : (r0 + rows_per_block < d ? r0 + rows_per_block : d);
   const int64_t rstep = strided ? int64_t(gridDim.x) * ROWS : ROWS;
-  // Tiles are prefetched STG ahead into registers: the narrow shapes (one or two
-  // 16 x 16 output tiles, the inner solve's projections) hold few bytes per
-  // tile, and with one tile in flight a workgroup waited on memory latency
-  constexpr int STG = MT * NT <= 2 ? 3 : 1;
-  double2 va[STG][PMAX][2], vb[STG][PMAX][2];
-  auto fetch = [&](int sg, int64_t rb) {
+  // One tile in flight per workgroup while the matrix cores work on the
+  // previous one: three register stages (more bytes in flight per workgroup)
+  // measured 4-8% slower on the narrow shapes (fewer resident workgroups)
+  double2 va[PMAX][2], vb[PMAX][2];
+  auto fetch = [&](int64_t rb) {
 #pragma unroll
     for (int p = 0; p < PMAX; ++p) {
       const int64_t row = rb + 2 * ((tid + kGramThreads * p) % HALF);
-      va[sg][p][0] = va[sg][p][1] = vb[sg][p][0] = vb[sg][p][1] = make_double2(0.0, 0.0);
+      va[p][0] = va[p][1] = vb[p][0] = vb[p][1] = make_double2(0.0, 0.0);
       if (pa[p] != nullptr && row < r1) {
         const bool two = row + 1 < r1;
-        ld_col2(pa[p], pf[p], row, two, va[sg][p][0], va[sg][p][1]);
-        if (pb[p] != nullptr) ld_col2(pb[p], pf[p], row, two, vb[sg][p][0], vb[sg][p][1]);
+        ld_col2(pa[p], pf[p], row, two, va[p][0], va[p][1]);
+        if (pb[p] != nullptr) ld_col2(pb[p], pf[p], row, two, vb[p][0], vb[p][1]);
       }
     }
   };
-  auto store = [&](int sg) {
+  auto store = [&]() {
 #pragma unroll
     for (int p = 0; p < PMAX; ++p) {
       if (pa[p] == nullptr) continue;  // beyond the loads, or supplied through a bslot
@@ -801,21 +808,25 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (pb[p] == nullptr) {
-          tile[(r + h) * LD + slot] = va[sg][p][h];
+          tile[(r + h) * LD + slot] = va[p][h];
         } else {
           const ColDesc &dc = descs[c];
           double2 o;
-          o.x = va[sg][p][h].x - (dc.sig_re * vb[sg][p][h].x - dc.sig_im * vb[sg][p][h].y);
-          o.y = va[sg][p][h].y - (dc.sig_re * vb[sg][p][h].y + dc.sig_im * vb[sg][p][h].x);
+          o.x = va[p][h].x - (dc.sig_re * vb[p][h].x - dc.sig_im * vb[p][h].y);
+          o.y = va[p][h].y - (dc.sig_re * vb[p][h].y + dc.sig_im * vb[p][h].x);
           tile[(r + h) * LD + slot] = o;
-          if (dc.bslot >= 0) tile[(r + h) * LD + dc.bslot] = vb[sg][p][h];
+          if (dc.bslot >= 0) tile[(r + h) * LD + dc.bslot] = vb[p][h];
         }
       }
     }
   };
-  auto multiply = [&]() {
+  if (r0 < r1) fetch(r0);
+  for (int64_t rb = r0; rb < r1; rb += rstep) {
+    store();
+    __syncthreads();
+    if (rb + rstep < r1) fetch(rb + rstep);
 #pragma unroll 2
-    for (int k0 = 0; k0 < ROWS; k0 += 4) {
+    for (int k0 = 4 * ks; k0 < ROWS; k0 += 4 * KS) {
       const double2 *trow = tile + (k0 + lk) * LD;
 #pragma unroll
       for (int q = 0; q < QMAX; ++q) {
@@ -829,27 +840,38 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
         acci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.y, bv.x, acci[q], 0, 0, 0);
       }
     }
-  };
+    __syncthreads();
+  }
+  if constexpr (KS > 1) {
+    // the waves with ks > 0 hand their sums to the tile's ks = 0 wave (the last
+    // __syncthreads of the loop has freed the row tile)
+    double *red = reinterpret_cast<double *>(tile);
+    const int t = wave / KS;
+    if (ks > 0 && tv[0]) {
+      double *slot = red + size_t(t * (KS - 1) + ks - 1) * 512;
 #pragma unroll
-  for (int sg = 0; sg < STG; ++sg)
-    if (r0 + sg * rstep < r1) fetch(sg, r0 + sg * rstep);
-  for (int64_t rb = r0; rb < r1; rb += STG * rstep) {
+      for (int v = 0; v < 4; ++v) {
+        slot[v * 64 + lane] = accr[0][v];
+        slot[(4 + v) * 64 + lane] = acci[0][v];
+      }
+    }
+    __syncthreads();
+    if (ks == 0 && tv[0]) {
 #pragma unroll
-    for (int sg = 0; sg < STG; ++sg) {
-      const int64_t rt = rb + sg * rstep;  // workgroup-uniform
-      if (rt < r1) {
-        store(sg);
-        __syncthreads();
-        if (rt + STG * rstep < r1) fetch(sg, rt + STG * rstep);
-        multiply();
-        __syncthreads();
+      for (int k = 1; k < KS; ++k) {
+        const double *slot = red + size_t(t * (KS - 1) + k - 1) * 512;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          accr[0][v] += slot[v * 64 + lane];
+          acci[0][v] += slot[(4 + v) * 64 + lane];
+        }
       }
     }
   }
   double2 *out = partial + size_t(blockIdx.x) * kl * kr;
 #pragma unroll
   for (int q = 0; q < QMAX; ++q) {
-    if (!tv[q]) continue;
+    if (!tv[q] || ks != 0) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int i = 16 * tm[q] + 4 * v + lk, j = 16 * tn[q] + li;
@@ -1494,8 +1516,8 @@ void gram_grids_nb() {
 }
 
 // Round-robin tiles (the default) or, with DEX_GRAM_STRIDE=0 (read once), one
-// contiguous row range per workgroup.  Measured at NN=21, block 16: the same
-// for complex128 columns, 13% less Gram time for complex64 ones.
+// contiguous row range per workgroup.  Measured on whole NN=21 block-16 solves:
+// Gram time within 1% either way (complex128 and complex64 columns).
 bool gram_stride_on() {
   static const bool on = [] {
     const char *v = std::getenv("DEX_GRAM_STRIDE");

@@ -39,6 +39,14 @@ class Request:
     inner_block: int = dataclasses.field(compare=False, default=0)
     guard: int = dataclasses.field(compare=False, default=0)
     ns: int = dataclasses.field(compare=False, default=1)            # GPLMR_NS
+    # admission reserves the footprint of this precision's storage (None: the request's own), so
+    # that two forms of a stream can be admitted identically
+    reserve_precision: str | None = dataclasses.field(compare=False, default=None)
+
+    def need(self) -> int:
+        """Device bytes the server reserves for this request."""
+        return footprint_bytes(self.n, self.nev, self.reserve_precision or self.precision, self.ns,
+                               self.inner_iter, self.inner_block, self.guard)
 
 
 @dataclasses.dataclass
@@ -283,8 +291,7 @@ class Server:
                     pick = 0
                     if self.policy == "smallest":
                         # the smallest waiting request that fits now (else the smallest, which waits)
-                        needs = [footprint_bytes(w[2].n, w[2].nev, w[2].precision, w[2].ns, w[2].inner_iter,
-                                                 w[2].inner_block, w[2].guard) for w in waiting]
+                        needs = [w[2].need() for w in waiting]
                         order = sorted(range(len(waiting)), key=lambda k: (needs[k], waiting[k][:2]))
                         fits = [k for k in order if self._in_use + needs[k] <= self.cap or needs[k] > self.cap]
                         pick = fits[0] if fits else order[0]
@@ -298,8 +305,7 @@ class Server:
                             heapq.heapify(waiting)
 
                     _, _, req = waiting[pick]
-                    need = footprint_bytes(req.n, req.nev, req.precision, req.ns, req.inner_iter,
-                                           req.inner_block, req.guard)
+                    need = req.need()
                     if need > self.cap:
                         take()
                         results[req.rid] = Result(req.rid, False, None, None, 0, 0, 0.0, 0.0,

@@ -121,3 +121,11 @@ def test_smallest_first_policy_admits_by_footprint():
             reqs, realtime=False, on_result=lambda r, q: seen.append(r.rid))
         assert all(r.ok for r in out)
         assert seen == want, (policy, seen)
+
+
+def test_reserve_precision_sets_the_admitted_footprint():
+    r = Request(arrival=0.0, rid=0, n=10, delta=0.4, theta=0.3, sigma=-0.7 + 0j, nev=16, tol=1e-7,
+                precision="mixed", ns=1, inner_iter=4, inner_block=4)
+    assert r.need() == footprint_bytes(10, 16, "mixed", 1, 4, 4, 0)
+    r.reserve_precision = "fp64"
+    assert r.need() == footprint_bytes(10, 16, "fp64", 1, 4, 4, 0) > footprint_bytes(10, 16, "mixed", 1, 4, 4, 0)
