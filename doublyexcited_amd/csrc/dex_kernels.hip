@@ -726,6 +726,7 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
   // end: with one wave per tile, a one-tile Gram ran all its MFMAs on one SIMD
   // of four and every workgroup waited on them
   constexpr int KS = MT * NT == 1 ? 4 : (MT * NT == 2 ? 2 : 1);
+  static_assert(gram_mfma_rows(MT, NT, TALL) % (4 * KS) == 0, "every wave takes whole row steps");
   // loads per thread per tile, each an adjacent row pair of one column (16-byte
   // accesses for complex64 and complex128 alike): every thread issues all of
   // them before any wait, and the next tile's while the matrix cores work on
@@ -825,8 +826,10 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
     store();
     __syncthreads();
     if (rb + rstep < r1) fetch(rb + rstep);
-#pragma unroll 2
-    for (int k0 = 4 * ks; k0 < ROWS; k0 += 4 * KS) {
+    // this wave's row steps: k0 = 4 (ks + KS s)
+#pragma unroll
+    for (int st = 0; st < ROWS / (4 * KS); ++st) {
+      const int k0 = 4 * (ks + KS * st);
       const double2 *trow = tile + (k0 + lk) * LD;
 #pragma unroll
       for (int q = 0; q < QMAX; ++q) {
@@ -877,6 +880,137 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_mfma(const ColDesc *__res
       const int i = 16 * tm[q] + 4 * v + lk, j = 16 * tn[q] + li;
       if (i < kl && j < kr) out[i + size_t(kl) * j] = make_double2(accr[q][v], acci[q][v]);
     }
+  }
+}
+
+// Partial Gram G = L^H R for narrow shapes (kl <= 8 M8 <= 32, kr <= 8: the
+// inner solve's projections of 4-column groups) with the real and imaginary
+// parts as separate MFMA rows and columns: A = [Re L_c; Im L_c] (16 rows for 8
+// left columns), B = [Re R_r, Im R_r] (16 columns for 8 right columns), so one
+// v_mfma_f64_16x16x4f64 per 4 rows forms all four real products of an 8 x 8
+// complex block, O = [Lr Rr, Lr Ri; Li Rr, Li Ri], and G(c, r) =
+// O(c, r) + O(c+8, r+8) + i (O(c, r+8) - O(c+8, r)) at the end.  The 16 x 16
+// complex tiles of k_gram_mfma take 4 MFMAs for any kl <= 16, kr <= 16; this
+// layout takes ceil(kl / 8).  Rows are staged in LDS as in k_gram_mfma (one
+// row tile per workgroup, the next prefetched, tiles round robin over the
+// grid); KS waves share each 8 x 8 block and split its row steps.  Right
+// columns are unshared (share 0) or all shared with the first left columns
+// (share == kr).  Layout of the partials as k_gram_mfma's.
+template <int M8, int ROWS>
+__global__ __launch_bounds__(kGramThreads) void k_gram_pk(const ColDesc *__restrict__ descs, int kl, int kr,
+                                                          int share, int64_t d, double2 *__restrict__ partial) {
+  constexpr int LD = 8 * M8 + 8 + 1;
+  constexpr int KS = M8 == 1 ? 4 : (M8 == 2 ? 2 : 1);
+  static_assert(ROWS % (4 * KS) == 0, "every wave takes whole row steps");
+  constexpr int HALF = ROWS / 2;
+  constexpr int PMAX = (HALF * (8 * M8 + 8) + kGramThreads - 1) / kGramThreads;
+  // the row tile, reused after the loop for the waves' 16 x 16 sums (256 doubles each)
+  constexpr int TILE2 = ROWS * LD > M8 * KS * 128 ? ROWS * LD : M8 * KS * 128;
+  __shared__ double2 tile[TILE2];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int t = KS > 1 ? wave / KS : wave;  // this wave's 8-column block of L
+  const int ks = KS > 1 ? wave % KS : 0;    // and its share of the row steps
+  const bool tv = t < M8 && 8 * t < kl;
+  const int nload = kl + (kr - share);
+  dex_v4d acc = dex_v4d{0.0, 0.0, 0.0, 0.0};
+  const void *pa[PMAX];
+  const void *pb[PMAX];
+  int pf[PMAX];
+#pragma unroll
+  for (int p = 0; p < PMAX; ++p) {
+    const int e = tid + kGramThreads * p;
+    pa[p] = nullptr;
+    pb[p] = nullptr;
+    pf[p] = 0;
+    if (e < HALF * nload) {
+      const ColDesc &dc = descs[e / HALF];
+      pa[p] = dc.a;
+      pb[p] = dc.b;
+      pf[p] = dc.fmt;
+    }
+  }
+  const int64_t r0 = int64_t(blockIdx.x) * ROWS;
+  const int64_t rstep = int64_t(gridDim.x) * ROWS;
+  double2 va[PMAX][2], vb[PMAX][2];
+  auto fetch = [&](int64_t rb) {
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+      const int64_t row = rb + 2 * ((tid + kGramThreads * p) % HALF);
+      va[p][0] = va[p][1] = vb[p][0] = vb[p][1] = make_double2(0.0, 0.0);
+      if (pa[p] != nullptr && row < d) {
+        const bool two = row + 1 < d;
+        ld_col2(pa[p], pf[p], row, two, va[p][0], va[p][1]);
+        if (pb[p] != nullptr) ld_col2(pb[p], pf[p], row, two, vb[p][0], vb[p][1]);
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+      if (pa[p] == nullptr) continue;
+      const int e = tid + kGramThreads * p;
+      const int r = 2 * (e % HALF), c = e / HALF;
+      const int slot = c < kl ? c : 8 * M8 + (c - kl);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double2 o = va[p][h];
+        if (pb[p] != nullptr) {
+          const ColDesc &dc = descs[c];
+          o.x -= dc.sig_re * vb[p][h].x - dc.sig_im * vb[p][h].y;
+          o.y -= dc.sig_re * vb[p][h].y + dc.sig_im * vb[p][h].x;
+        }
+        tile[(r + h) * LD + slot] = o;
+      }
+    }
+  };
+  // this lane's A row (a part of left column 8 t + (li & 7)) and B column (a
+  // part of right column li & 7; shared right columns are the first left ones)
+  const int acol = 8 * t + (li & 7);
+  const int rj = li & 7;
+  const int bcol = rj < share ? rj : 8 * M8 + (rj - share);
+  const bool im = li >= 8;
+  if (r0 < d) fetch(r0);
+  for (int64_t rb = r0; rb < d; rb += rstep) {
+    store();
+    __syncthreads();
+    if (rb + rstep < d) fetch(rb + rstep);
+    if (tv) {
+#pragma unroll
+      for (int st = 0; st < ROWS / (4 * KS); ++st) {
+        const double2 *trow = tile + (4 * (ks + KS * st) + lk) * LD;
+        const double2 av = trow[acol];
+        const double2 bv = trow[bcol];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(im ? av.y : av.x, im ? bv.y : bv.x, acc, 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // every wave's 16 x 16 sums to LDS (the loop's last __syncthreads freed the
+  // tile); element (i, j) of the block, i = 4 v + lk, j = li, sits at
+  // [v * 64 + lane] of the wave's 256 doubles
+  double *red = reinterpret_cast<double *>(tile);
+  if (tv) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[size_t(t * KS + ks) * 256 + v * 64 + lane] = acc[v];
+  }
+  __syncthreads();
+  double2 *out = partial + size_t(blockIdx.x) * kl * kr;
+  for (int e = tid; e < M8 * 64; e += kGramThreads) {
+    const int tb = e / 64, c = (e % 64) / 8, r = e % 8;
+    const int i = 8 * tb + c;
+    if (i >= kl || r >= kr) continue;
+    auto O = [&](int k, int ii, int jj) {
+      return red[size_t(tb * KS + k) * 256 + (ii / 4) * 64 + (jj + 16 * (ii % 4))];
+    };
+    double gr = 0.0, gi = 0.0;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      gr += O(k, c, r) + O(k, c + 8, r + 8);
+      gi += O(k, c, r + 8) - O(k, c + 8, r);
+    }
+    out[i + size_t(kl) * r] = make_double2(gr, gi);
   }
 }
 
@@ -1492,6 +1626,45 @@ int gram_rows_choice() {
   return rows;
 }
 
+// The packed narrow Gram (on unless DEX_GRAM_PK=0, read once).
+bool gram_pk_on() {
+  static const bool on = [] {
+    const char *v = std::getenv("DEX_GRAM_PK");
+    return !(v && std::atoi(v) == 0);
+  }();
+  return on;
+}
+
+constexpr int kPkRows = 32;
+
+template <int M8>
+int gram_pk_grid() {
+  static std::atomic<int> cached{0};
+  int grid = cached.load(std::memory_order_acquire);
+  if (grid != 0) return grid;
+  std::lock_guard<std::mutex> lk(g_setup_mu);
+  grid = cached.load(std::memory_order_acquire);
+  if (grid != 0) return grid;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(&k_gram_pk<M8, kPkRows>), kGramThreads, 0) != hipSuccess ||
+      cus < 1 || per_cu < 1)
+    return kGramBlocks;
+  grid = cus * per_cu < kGramBlocks ? cus * per_cu : kGramBlocks;
+  cached.store(grid, std::memory_order_release);
+  return grid;
+}
+
+template <int M8>
+int gram_pk_launch(const ColDesc *descs, int kl, int kr, int share, int64_t d, double2 *partial, hipStream_t s) {
+  const int grid = gram_pk_grid<M8>();
+  hipLaunchKernelGGL((k_gram_pk<M8, kPkRows>), dim3(grid), dim3(kGramThreads), 0, s, descs, kl, kr, share, d,
+                     partial);
+  return grid;
+}
+
 template <int MA>
 void gram_grids_nb() {
   if (gram_rows_choice() == 64) {
@@ -1572,6 +1745,19 @@ hipError_t launch_gram(const ColDesc *descs, int kl, int kr, int share, int64_t 
   const int ma = (kl + 15) / 16, nb = (kr + 15) / 16;
   double2 *pp = reinterpret_cast<double2 *>(partial);
   int nparts = 0;
+  if (kr <= 8 && kl <= 32 && (share == 0 || share == kr) && gram_pk_on()) {
+    switch ((kl + 7) / 8) {
+      case 1: nparts = gram_pk_launch<1>(descs, kl, kr, share, d, pp, s); break;
+      case 2: nparts = gram_pk_launch<2>(descs, kl, kr, share, d, pp, s); break;
+      case 3: nparts = gram_pk_launch<3>(descs, kl, kr, share, d, pp, s); break;
+      default: nparts = gram_pk_launch<4>(descs, kl, kr, share, d, pp, s); break;
+    }
+    const int nelem = kl * kr;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((nelem + 3) / 4), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(partial), nparts, nelem,
+                       reinterpret_cast<double2 *>(out));
+    return hipGetLastError();
+  }
   switch (ma) {
     case 1: nparts = gram_launch_nb<1>(nb, descs, kl, kr, share, d, pp, s); break;
     case 2: nparts = gram_launch_nb<2>(nb, descs, kl, kr, share, d, pp, s); break;
@@ -1753,6 +1939,10 @@ hipError_t prepare_kernels() {
   gram_grids_nb<2>();
   gram_grids_nb<3>();
   gram_grids_nb<4>();
+  (void)gram_pk_grid<1>();
+  (void)gram_pk_grid<2>();
+  (void)gram_pk_grid<3>();
+  (void)gram_pk_grid<4>();
   return ok ? hipSuccess : hipErrorInvalidDeviceFunction;
 }
 
