@@ -28,7 +28,7 @@ def short(name: str) -> str:
 # launch is the MFMA partial kernel plus its partial-sum reduction, a residual
 # step is its kernel plus a reduction, both counted once.
 CLASSES = (
-    ("k_gram_mfma", "gram", True), ("k_reduce_partials", "gram", False),
+    ("k_gram_mfma", "gram", True), ("k_gram_pk", "gram", True), ("k_reduce_partials", "gram", False),
     ("k_combine", "combine", True),
     ("k_pass1", "pass1", True), ("k_symadd", "symadd", True), ("k_pass2", "pass2", True),
     ("k_coldots", "coldots", True), ("k_resid", "coldots", True), ("k_reduce_real", "coldots", False),

@@ -27,7 +27,7 @@ CLASSES = {
     "pass1": ("k_pass1",),
     "pass2": ("k_pass2",),
     "symadd": ("k_symadd",),
-    "gram": ("k_gram_mfma", "k_reduce_partials"),
+    "gram": ("k_gram_mfma", "k_gram_pk", "k_reduce_partials"),
     "combine": ("k_combine",),
     "coldots": ("k_coldots", "k_resid", "k_reduce_real"),
     "generic": ("k_axis", "k_potential"),
