@@ -92,3 +92,24 @@ def test_stop_ends_a_solve_at_an_iteration_boundary(dex):
         r = g.gplmr_eig_device(x, sigma, 4, ns=1, maxit=100000, tol=1e-12, symmetry=1)
         t.join()
         assert 0 < r.iterations < 100000 and time.time() - t0 < 60.0
+
+
+def test_results_alone_equal_results_in_a_mix_of_grids_11_to_21(dex):
+    """The same at the served stream's grid sizes: NN=11, 15 and 21 (block 4,
+    a few iterations each, so the test stays short) alone and together; a
+    request's result does not depend on what else is on the device."""
+    from doublyexcited_amd.serve import Request, Server
+
+    print("case: serve alone vs mix, NN=11/15/21", flush=True)
+    specs = [(5, 0.5, 0.30, 6), (10, 0.42, 0.39, 3), (7, 0.45, 0.35, 4)]
+    reqs = [Request(arrival=0.0, rid=i, n=n, delta=dl, theta=th, sigma=-0.69 - 0.002j, nev=4, tol=1e-7,
+                    maxit=its, seed=31 + i, inner_iter=4, inner_block=4, ns=1)
+            for i, (n, dl, th, its) in enumerate(specs)]
+    alone = [Server(slots=1).run([r], realtime=False)[0] for r in reqs]
+    mixed = Server(slots=3).run(reqs, realtime=False)
+    for a, b in zip(alone, mixed):
+        assert a.ok and b.ok, (a.error, b.error)
+        assert a.iterations == b.iterations
+        np.testing.assert_allclose(b.e, a.e, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(b.residuals, a.residuals, rtol=0, atol=1e-12)
+    assert [r.nn for r in mixed] == [11, 21, 15]
