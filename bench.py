@@ -83,11 +83,13 @@ WORKLOADS = {
                            desc="He, NN=11 (D=1.77M), delta 0.5, theta 0.3, 8 eigenpairs nearest "
                                 "sigma -0.78-0.002i (block 8 + 2 guard columns), tol 1e-8, "
                                 "NS 4, inner solve order 8"),
-    # the drop-in FP64 GPU configuration (a solve takes many minutes)
+    # the drop-in FP64 GPU configuration (a solve takes about 10 minutes: --steps 1 --warmup 0)
     "he-nn21-block16": dict(n=10, delta=0.4, theta=0.35, nev=16, sigma=complex(-0.70, -0.001), tol=1e-9,
-                            seed=1002, ns=1, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
+                            seed=1002, ns=2, inner=4, inner_block=4, guard=0, lock=True, maxit=800,
+                            concurrency=1,
                             desc="He, NN=21 (D=85.8M), delta 0.4, theta 0.35, block 16, "
-                                 "sigma -0.70-0.001i, tol 1e-9, inner solve order 4 on 4-column groups"),
+                                 "sigma -0.70-0.001i, tol 1e-9, NS 2, inner solve order 4 on 4-column groups, "
+                                 "converged columns locked"),
     # theta-trajectory scan: theta evenly spaced over [0.20, 0.50], NN cycling 11, 13, 15, 17;
     # each point warm-started from the previous one (SIXD_FIX across grids); a step is one point
     "he-theta-scan": dict(n=5, delta=0.5, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-8,
@@ -127,7 +129,8 @@ def solve(torch, slot: Slot, w: dict, x, timing: int, inner_fp32: bool = False) 
         t0 = time.time()
         r = g.gplmr_eig_device(x, w["sigma"], w["nev"], ns=w["ns"], maxit=w["maxit"], tol=w["tol"],
                                inner_iter=w["inner"], inner_block=w["inner_block"], guard=w["guard"],
-                               symmetry=1, fp32_tol=max(w["tol"], 1e-5), inner_fp32=inner_fp32)
+                               lock=w.get("lock", False), symmetry=1, fp32_tol=max(w["tol"], 1e-5),
+                               inner_fp32=inner_fp32)
         lat = time.time() - t0
         stats = g.kernel_stats() if timing else {}
         g.set_kernel_timing(False)
