@@ -148,8 +148,8 @@ def _fortran_type_fields(text, name):
 
 
 def test_struct_mirrors_match_the_header():
-    """ctypes mirrors and the Fortran ISO_C_BINDING types (INTEGRATION.md and
-    the compiled Fortran check) list the header's fields in the header's order:
+    """ctypes mirrors and the Fortran ISO_C_BINDING types (the binding notes at
+    the repository root and the compiled Fortran check) list the header's fields in the header's order:
     a missing trailing field makes dex_solve_opts_default write past the
     caller's object."""
     from pathlib import Path

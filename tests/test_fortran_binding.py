@@ -1,4 +1,4 @@
-"""The Fortran ISO_C_BINDING interface shown in INTEGRATION.md compiles with
+"""The Fortran ISO_C_BINDING interface of the binding notes compiles with
 flang, links against libdex.so and calls host-only entry points (no GPU)."""
 import shutil
 import subprocess
