@@ -306,6 +306,14 @@ int apply_h(dex_ctx *c, const Dbl2 *x, Dbl2 *y, bool single, int sym = 0, bool p
   }
   KTimer kt(c, DEX_K_GENERIC, 32.0 * double(c->d), (24.0 * c->nn + 14.0) * double(c->d));
   DEX_HIP(dex::launch_matvec_generic(c->nn, x, y, c->d_vn, c->d_tinv, c->coef, s));
+  // The general operator carries the rounding-level exchange-odd (even) part of
+  // an exchange-even (odd) x into y, where the transpose-add kernels, which the
+  // mixed form applies to its complex64 blocks, drop it: in a solve that mixes
+  // the two, AX then disagrees with the images of the search blocks in that
+  // part, and the iteration drifts away from the wanted levels after tens of
+  // iterations (seen at NN=25, where FP64 has only these kernels).  Projecting y
+  // keeps both paths on the exchange-symmetric subspace.
+  if (sym != 0) DEX_HIP(dex::launch_sym_project(y, c->m, sym > 0 ? 1 : -1, s));
   return 0;
 }
 

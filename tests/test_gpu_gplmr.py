@@ -383,3 +383,23 @@ def test_kernel_statistics_every_and_sampled(dex):
         assert 0.5 * b1 < b4 < 2.0 * b1, cls
         assert s1[cls][0] > 0 and s4[cls][0] > 0
 
+
+
+def test_mixed_solve_where_fp64_has_only_the_generic_kernels(dex):
+    """NN=25 (no FP64 LDS cube fits, so the mixed form's FP64 AX comes from
+    the generic kernels while its complex64 blocks use the exchange
+    transpose-add ones): after 80 iterations the returned pairs still sit at
+    the levels near sigma with FP64 residuals below 0.5.  Without the
+    exchange projection of the generic output, the rounding-level odd part
+    grew through the iteration and the returned pairs had |E| near 50-100 au
+    and residuals near 70."""
+    print("case: mixed NN=25 block 2, 80 iterations", flush=True)
+    sigma = -0.78 - 0.002j
+    with dex.HeliumGrid(12, 0.35, 0.3, precision="mixed") as g:
+        x = torch.empty(g.d * 2, dtype=torch.complex128, device="cuda")
+        g.fill_gaussian_device(x, 2, seed=1003, symmetry=1)
+        r = g.gplmr_eig_device(x, sigma, 2, ns=1, maxit=80, tol=1e-7, inner_iter=4, inner_block=4, symmetry=1,
+                               fp32_tol=1e-5)
+        res = g.residuals_device(x, r.e)
+    assert np.all(np.abs(r.e - sigma) < 0.5), r.e
+    assert np.all(res < 0.5), res
