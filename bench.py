@@ -106,8 +106,10 @@ WORKLOADS = {
                                "11/13/15/17, delta 0.5, 8 pairs nearest -0.78-0.002i (block 8 + 2 guard), "
                                "tol 1e-8, NS 3, inner order 8, warm starts"),
     # the mixed-precision configuration (FP64 basis: 2 restarts after the FP32 phase)
+    # FP64 blocks at NN=25 need 285 GB and its FP64 matvec has no exchange-symmetric kernels: mixed form only
     "he-nn25-block8": dict(n=12, delta=0.35, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-7,
-                           seed=1003, ns=1, inner=4, inner_block=4, guard=0, maxit=600, concurrency=1,
+                           seed=1003, ns=1, inner=4, inner_block=4, guard=0, lock=True, maxit=600, concurrency=1,
+                           mixed_only=True,
                            desc="He, NN=25 (D=244M), delta 0.35, theta 0.3, block 8, "
                                 "sigma -0.78-0.002i, tol 1e-7, inner solve order 4 on 4-column groups"),
 }
