@@ -514,6 +514,27 @@ def traffic_run_served(out_path: str, steps: int, form: str, same_plan: bool = T
     return 0
 
 
+def match_forms(res64, resmx, tol: float, window: float = 1e-4):
+    """Eigenvalues of the same request in both forms: each converged mixed pair
+    (residual <= tol) against the nearest converged FP64 pair of that request.
+    A pair with none within ``window`` au (its level did not converge in the
+    FP64 form) is counted, not compared.  Returns (|dE| list, unmatched count)."""
+    import numpy as np
+
+    de, unmatched = [], 0
+    for a, b in zip(res64, resmx):
+        if not (a.ok and b.ok) or a.residuals is None or b.residuals is None:
+            continue
+        e64 = np.asarray(a.e)[np.asarray(a.residuals) <= tol]
+        for z in np.asarray(b.e)[np.asarray(b.residuals) <= tol]:
+            dz = float(np.min(np.abs(e64 - z))) if len(e64) else np.inf
+            if dz <= window:
+                de.append(dz)
+            else:
+                unmatched += 1
+    return de, unmatched
+
+
 def latency_stats(lat) -> dict | None:
     import numpy as np
 
@@ -593,20 +614,7 @@ def run_served(args) -> int:
     if "mixed" in out:
         resmx, wallmx = out["mixed"][1], out["mixed"][2]
         pairsmx = sum(r.converged_pairs for r in resmx)
-        # eigenvalues of the same request in both forms: each converged mixed pair against the nearest
-        # converged FP64 pair of that request; a pair with none within 1e-4 au (its level was not converged
-        # in the FP64 form within the horizon) is counted, not compared
-        de, unmatched = [], 0
-        for a, b in zip(res64, resmx):
-            if not (a.ok and b.ok) or a.residuals is None or b.residuals is None:
-                continue
-            e64 = np.asarray(a.e)[np.asarray(a.residuals) <= sv["tol"]]
-            for z in np.asarray(b.e)[np.asarray(b.residuals) <= sv["tol"]]:
-                dz = float(np.min(np.abs(e64 - z))) if len(e64) else np.inf
-                if dz <= 1e-4:
-                    de.append(dz)
-                else:
-                    unmatched += 1
+        de, unmatched = match_forms(res64, resmx, sv["tol"])
         summary["mixed"] = form_summary(*out["mixed"])
         summary["mixed"].update({
             "fp64_refreshes_mean": float(np.mean([r.fp64_restarts for r in resmx if r.ok])),
