@@ -47,7 +47,12 @@ Also measured in the same run:
                 HAMMULT timed per grid and numpy's block products timed at a
                 representative shape, applied to the FP64 solves' own counts
                 (matvecs per grid, block-kernel bytes), i.e. timed components of
-                a bounded sample scaled to the stream.
+                a bounded sample scaled to the stream.  ``--cpu-solve-sample
+                RID`` (host only, no GPU) times instead the oracle's whole GPLMR
+                solve of the stream's request RID for ``--cpu-solve-iterations``
+                iterations: the oracle orthogonalises the inner solve column by
+                column as the reference does, which the blocked model does not
+                price.
 
 The other workloads (``--workload``) are the single-configuration solves
 (NN=11 block 8, NN=21 block 16, NN=25 block 8) and the theta scan.
@@ -90,8 +95,6 @@ WORKLOADS = {
                             desc="He, NN=21 (D=85.8M), delta 0.4, theta 0.35, block 16, "
                                  "sigma -0.70-0.001i, tol 1e-9, NS 2, inner solve order 4 on 4-column groups, "
                                  "converged columns locked"),
-    # theta-trajectory scan: theta evenly spaced over [0.20, 0.50], NN cycling 11, 13, 15, 17;
-    # each point warm-started from the previous one (SIXD_FIX across grids); a step is one point
     # the same in the mixed form, with 2 guard columns (179 GB; FP64 blocks would need 306 GB)
     "he-nn21-block16-mixed": dict(n=10, delta=0.4, theta=0.35, nev=16, sigma=complex(-0.70, -0.001), tol=1e-9,
                                   seed=1002, ns=2, inner=4, inner_block=4, guard=2, lock=True, maxit=700,
@@ -99,6 +102,8 @@ WORKLOADS = {
                                   desc="He, NN=21 (D=85.8M), delta 0.4, theta 0.35, block 16 + 2 guard columns, "
                                        "sigma -0.70-0.001i, tol 1e-9, NS 2, inner solve order 4 on 4-column "
                                        "groups, converged columns locked, mixed form"),
+    # theta-trajectory scan: theta evenly spaced over [0.20, 0.50], NN cycling 11, 13, 15, 17;
+    # each point warm-started from the previous one (SIXD_FIX across grids); a step is one point
     "he-theta-scan": dict(n=5, delta=0.5, theta=0.3, nev=8, sigma=complex(-0.78, -0.002), tol=1e-8,
                           seed=1004, ns=3, inner=8, inner_block=0, guard=2, maxit=800, concurrency=1,
                           scan=dict(points=64, theta=(0.20, 0.50), nn=(11, 13, 15, 17)),
@@ -379,12 +384,74 @@ def _roofline_from_results(results, sample: int) -> dict:
     return algorithmic_roofline(rows, sample)
 
 
+def oracle_solve_sample(q, iterations: int = 1, seed: int = 0) -> dict:
+    """Times the oracle's FP64 solve on the host for request q: GPLMR_EIG as
+    the oracle restates it in numpy (gplmr.F90:9-386), with the C HAMMULT
+    (hammod.F90:19-67) as MATBLOCK, on q's grid with q's block, guard columns,
+    NS and inner order (PRECBLOCK_BASIC on the whole block, whose Gram-Schmidt
+    runs column by column as harmonic_ritz_small.F90:28-81 does), from a
+    seeded exchange-symmetric Gaussian start block, for the given number of
+    iterations (fewer if it converges).  Returns the seconds of those
+    iterations and the part spent in the matvecs; setting up the grid's tables
+    is timed apart."""
+    import numpy as np
+
+    from oracle.c_oracle import CHammult
+    from oracle.gplmr import gplmr_eig
+    from oracle.hamiltonian import Hamiltonian
+
+    if q.inner_block not in (0, q.nev + q.guard):
+        raise ValueError("the oracle's inner solve works on the whole block")
+    t0 = time.time()
+    h = Hamiltonian(q.n, q.delta, q.theta)
+    hm = CHammult(h, c_potential=True)
+    nb = q.nev + q.guard
+    rng = np.random.default_rng(seed)
+    x0 = np.empty((h.d, nb), dtype=np.complex128, order="F")
+    for j in range(nb):
+        g = (rng.standard_normal((h.m, h.m)) + 1j * rng.standard_normal((h.m, h.m))) / np.sqrt(2.0)
+        x0[:, j] = ((g + g.T) * 0.5).reshape(-1)
+        del g
+    t_setup = time.time() - t0
+    mv = [0, 0.0]
+
+    def amul(b):
+        t = time.time()
+        y = hm(b)
+        mv[0] += b.shape[1]
+        mv[1] += time.time() - t
+        return y
+
+    t1 = time.time()
+    r = gplmr_eig(amul, x0, q.sigma, nb, ns=q.ns, maxit=iterations, tol=q.tol, inner_iter=q.inner_iter,
+                  nconv=q.nev)
+    return {"rid": q.rid, "nn": 2 * q.n + 1, "block": q.nev, "guard": q.guard, "ns": q.ns,
+            "inner": q.inner_iter, "iterations": r.iters, "matvecs": mv[0], "seconds": time.time() - t1,
+            "matvec_seconds": mv[1], "setup_seconds": t_setup, "converged": bool(r.converged),
+            "threads": CHammult.max_threads()}
+
+
+def run_cpu_solve_sample(rid: int, iterations: int) -> int:
+    """Host only: the stream's request rid (the FP64 form's plan) solved by the
+    oracle for a few iterations; one JSON line."""
+    from doublyexcited_amd.serve import poisson_stream
+
+    sv = SERVED
+    reqs = poisson_stream(sv["rate"], max(rid + 1, 20), sv["seed"], precision="fp64", plan=True, cap=sv["cap"],
+                          maxit=sv["maxit"], tol=sv["tol"])
+    print(json.dumps(oracle_solve_sample(reqs[rid], iterations)))
+    return 0
+
+
 def served_cpu_baseline(res64, budget_s: float = 25.0) -> dict:
     """The oracle's FP64 path on the host: the C HAMMULT timed at every grid of
     the stream and numpy's block products (Gram X^H Y and update X C, the
     operations of the block kernels) timed at NN=15 with 16 columns; each FP64
     request's CPU time is its matvec count x the HAMMULT time of its grid plus
-    its block kernels' algorithmic bytes / the numpy block-product bandwidth."""
+    its block kernels' algorithmic bytes / the numpy block-product bandwidth.
+    This prices a CPU program that runs the library's blocked algorithm; the
+    oracle's own solve, whose inner solve orthogonalises column by column as
+    the reference does, takes longer (``oracle_solve_sample``)."""
     import numpy as np
 
     from oracle.c_oracle import CHammult
@@ -695,12 +762,17 @@ def main() -> int:
                          "kernel-statistics profile runs); prints its roofline object")
     ap.add_argument("--traffic-run", metavar="STATS_JSON",
                     help="only run the program the HBM-traffic counter passes profile")
+    ap.add_argument("--cpu-solve-sample", type=int, metavar="RID", default=-1,
+                    help="served stream, host only: time the oracle's GPLMR solve of request RID")
+    ap.add_argument("--cpu-solve-iterations", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: independent solves "
               "would shard across GPUs with no communication, and this build runs on one GPU")
         return 2
     if args.workload == "he-served-stream":
+        if args.cpu_solve_sample >= 0:
+            return run_cpu_solve_sample(args.cpu_solve_sample, args.cpu_solve_iterations)
         if args.traffic_run:
             return traffic_run_served(args.traffic_run, args.steps, args.forms.split(",")[-1], args.same_plan)
         if args.roofline_only:

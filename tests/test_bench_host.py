@@ -28,3 +28,25 @@ def test_latency_stats():
     assert bench.latency_stats([]) is None
     st = bench.latency_stats([1.0, 2.0, 3.0, 4.0])
     assert st["n"] == 4 and st["max"] == 4.0 and st["p50"] == 2.5 and 3.9 < st["p99"] <= 4.0
+
+
+def test_oracle_solve_sample_runs_the_requested_iterations():
+    """The host-only timing of the oracle's solve: one iteration of a tiny
+    request (NN=5, block 2 + 1 guard, NS 2, inner order 3) counts every
+    matvec and times them inside the iteration's seconds."""
+    import pytest
+
+    from doublyexcited_amd.serve import Request
+
+    q = Request(arrival=0.0, rid=3, n=2, delta=0.5, theta=0.3, sigma=-0.78 - 0.002j, nev=2, tol=1e-7,
+                inner_iter=3, guard=1, ns=2, maxit=40)
+    s = bench.oracle_solve_sample(q, iterations=1)
+    assert s["iterations"] == 1 and s["nn"] == 5 and s["block"] == 2 and s["guard"] == 1
+    # the start block (3 columns), then per iteration the inner solve on W and on each of the
+    # NS S blocks (2 order multiplications of <= 3 columns each: H V_k and (H - sigma) V_k)
+    # and H W, H S_l
+    assert 3 < s["matvecs"] <= 3 + (2 + 1) * (2 * 3 + 1) * 3
+    assert 0.0 <= s["matvec_seconds"] <= s["seconds"]
+    q.inner_block = 2  # 2-column groups: not what the oracle's inner solve restates
+    with pytest.raises(ValueError):
+        bench.oracle_solve_sample(q, iterations=1)
