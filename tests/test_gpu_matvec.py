@@ -145,6 +145,33 @@ def test_dots_match_numpy(dex):
     np.testing.assert_allclose(out, b.conj().T @ k, rtol=1e-12, atol=1e-9)
 
 
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_dots_device_over_the_gram_shapes(dex, n):
+    """MYDOTSUB on device columns over the shapes of both Gram kernels: the
+    packed real/imaginary one (kl <= 32, kr <= 8) and the 16 x 16 MFMA tiles,
+    one launch (<= 64 x 128) and chunked (beyond), on D = 729, 15625 and
+    117649 rows (none a multiple of the row tiles).  Each entry within
+    1e-13 ||b_i|| ||k_j|| of numpy's FP64 product (a bound that a dropped
+    row tile, a swapped real/imaginary part or a missing conjugate exceeds)."""
+    shapes = [(1, 1), (3, 5), (8, 8), (13, 7), (32, 8), (33, 8), (5, 9), (16, 16), (40, 70), (64, 128),
+              (70, 130)]
+    with dex.HeliumGrid(n, 0.5, 0.3) as g:
+        kl_max = max(s[0] for s in shapes)
+        kr_max = max(s[1] for s in shapes)
+        rng = np.random.default_rng(40 + n)
+        b = rng.standard_normal((kl_max, g.d)) + 1j * rng.standard_normal((kl_max, g.d))
+        k = rng.standard_normal((kr_max, g.d)) + 1j * rng.standard_normal((kr_max, g.d))
+        bt = torch.from_numpy(b).cuda()
+        kt = torch.from_numpy(k).cuda()
+        nb = np.linalg.norm(b, axis=1)
+        nk = np.linalg.norm(k, axis=1)
+        for kl, kr in shapes:
+            out = g.dots_device(bt[:kl].contiguous().view(-1), kt[:kr].contiguous().view(-1), kl, kr)
+            ref = b[:kl].conj() @ k[:kr].T
+            err = np.abs(out - ref) / np.outer(nb[:kl], nk[:kr])
+            assert err.max() <= 1e-13, (kl, kr, g.d, err.max())
+
+
 @pytest.mark.parametrize("n,sym", [(3, 1), (5, -1), (10, 1), (10, -1)])
 def test_exchange_symmetric_matvec_matches_general(dex, n, sym):
     """dex_matvec_sym_device on X = sym X^T agrees with the general HAMMULT and
