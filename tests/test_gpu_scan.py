@@ -48,3 +48,28 @@ def test_grid_change_through_sixd_fix(dex):
     out = theta_scan(pts, sigma=-0.78 - 0.002j, nev=4, tol=1e-8, inner_iter=4, guard=2, maxit=400)
     assert out[1].warm
     assert out[1].converged_pairs == 4
+
+
+def test_scan_at_the_scan_configuration_grids(dex):
+    """The theta scan's own settings (block 8 + 2 guard columns, NS 3, inner
+    order 8, tol 1e-8, delta 0.5, sigma -0.78-0.002i) on its first grids:
+    NN=11 at theta 0.30 and 0.31, then NN=13 at 0.31 through SIXD_FIX.  Each
+    warm point converges all 8 pairs to the eigenvalues a cold seeded start
+    reaches at the same point (within 1e-8 au), and the warm theta step takes
+    fewer iterations than the cold start."""
+    from doublyexcited_amd.scan import theta_scan
+
+    kw = dict(sigma=-0.78 - 0.002j, nev=8, tol=1e-8, ns=3, inner_iter=8, guard=2, maxit=400)
+    pts = [(5, 0.5, 0.30), (5, 0.5, 0.31), (6, 0.5, 0.31)]
+    print("case: warm scan NN=11, 11, 13", flush=True)
+    warm = theta_scan(pts, warm=True, **kw)
+    print("case: cold starts NN=11, 13 at theta 0.31", flush=True)
+    cold = theta_scan(pts[1:], warm=False, seed=91, **kw)
+    for p in warm + cold:
+        print(f"  NN={2 * p.n + 1} theta {p.theta} warm {p.warm}: {p.converged_pairs}/8 pairs, "
+              f"{p.iterations} iterations, {p.seconds:.1f} s", flush=True)
+    assert [p.warm for p in warm] == [False, True, True]
+    assert all(p.converged_pairs == 8 for p in warm + cold)
+    for w, c in ((warm[1], cold[0]), (warm[2], cold[1])):
+        assert match(w.e, c.e) < 1e-8 and match(c.e, w.e) < 1e-8
+    assert warm[1].iterations < cold[0].iterations
