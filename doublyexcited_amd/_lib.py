@@ -32,7 +32,7 @@ KERNEL_CLASSES = ("pass1", "pass2", "symadd", "gram", "combine", "coldots", "gen
 EXPORTS = [
     "dex_params_default", "dex_ctx_create", "dex_ctx_destroy", "dex_ctx_info", "dex_ctx_stop",
     "dex_get_kematrix", "dex_get_tinv", "dex_matvec", "dex_matvec_device", "dex_matvec_sym_device",
-    "dex_matvec_noalloc",
+    "dex_matvec_noalloc", "dex_matvec_graph_create", "dex_graph_launch", "dex_graph_destroy",
     "dex_dots",
     "dex_dots_device", "dex_solve_opts_default", "dex_gplmr_eig", "dex_gplmr_eig_device",
     "dex_fill_gaussian_device", "dex_sync", "dex_residuals_device", "dex_gplmr_footprint",
@@ -100,6 +100,10 @@ def load() -> C.CDLL:
         "dex_matvec_device": (C.c_int, [P, P, P, C.c_int32, C.c_int64, C.c_int32]),
         "dex_matvec_sym_device": (C.c_int, [P, P, P, C.c_int32, C.c_int64, C.c_int32, C.c_int32]),
         "dex_matvec_noalloc": (C.c_int, [P, P, P, C.c_int32, C.c_int64, C.c_int32, C.c_int32]),
+        "dex_matvec_graph_create": (C.c_int, [P, P, P, C.c_int32, C.c_int64, C.c_int32, C.c_int32,
+                                              C.POINTER(P)]),
+        "dex_graph_launch": (C.c_int, [P]),
+        "dex_graph_destroy": (C.c_int, [P]),
         "dex_dots": (C.c_int, [P, dp, dp, C.c_int64, C.c_int32, C.c_int32, dp]),
         "dex_dots_device": (C.c_int, [P, P, P, C.c_int64, C.c_int32, C.c_int32, dp]),
         "dex_solve_opts_default": (None, [C.POINTER(DexSolveOpts)]),

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -905,6 +906,66 @@ int dex_matvec_noalloc(dex_ctx *c, const void *in, void *out, int32_t howmany, i
     else
       DEX_HIP(dex::launch_pass2(c->nn, io, x, c->d_scratch, y, c->d_vn, c->d_tinv, c->coef, c->stream));
   }
+  return DEX_OK;
+}
+
+struct dex_graph {
+  dex_ctx *ctx;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+int dex_matvec_graph_create(dex_ctx *c, const void *in, void *out, int32_t howmany, int64_t checksize,
+                            int32_t single, int32_t symmetry, dex_graph **out_graph) {
+  if (!c || !out_graph) return fail(DEX_EINVAL, "null argument");
+  *out_graph = nullptr;
+  if (howmany < 1) return fail(DEX_EINVAL, "howmany < 1");
+  DEX_HIP(hipSetDevice(c->p.device));
+  // The kernels' attributes were set when the context was created, so the capture
+  // records kernel launches only: no allocation, copy, memset or synchronisation.
+  DEX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  const int st = dex_matvec_noalloc(c, in, out, howmany, checksize, single, symmetry);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+  if (st != DEX_OK) {  // its own message stands
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return hip_fail(ec, "hipStreamEndCapture");
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(ei, "hipGraphInstantiate");
+  }
+  auto *g = new (std::nothrow) dex_graph{c, graph, exec};
+  if (!g) {
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return fail(DEX_EOOM, "graph handle");
+  }
+  *out_graph = g;
+  return DEX_OK;
+}
+
+int dex_graph_launch(dex_graph *g) {
+  if (!g) return fail(DEX_EINVAL, "null graph");
+  DEX_HIP(hipSetDevice(g->ctx->p.device));
+  DEX_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+  return DEX_OK;
+}
+
+int dex_graph_destroy(dex_graph *g) {
+  if (!g) return DEX_OK;
+  DEX_HIP(hipSetDevice(g->ctx->p.device));
+  const hipError_t e1 = hipGraphExecDestroy(g->exec);
+  const hipError_t e2 = hipGraphDestroy(g->graph);
+  delete g;
+  if (e1 != hipSuccess) return hip_fail(e1, "hipGraphExecDestroy");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipGraphDestroy");
   return DEX_OK;
 }
 

@@ -81,6 +81,36 @@ class SolveResult:
     status: int
 
 
+class MatvecGraph:
+    """A captured block matvec of one HeliumGrid (``HeliumGrid.matvec_graph``);
+    holds its input and output tensors so their memory outlives the graph."""
+
+    def __init__(self, grid: "HeliumGrid", handle, tensors):
+        self._grid = grid
+        self._h = handle
+        self._tensors = tensors
+
+    def launch(self):
+        """Replays the matvec on the grid's stream, after the work torch has
+        queued that may still be writing the input."""
+        if not self._h.value:
+            raise ValueError("the graph is closed")
+        _dev(self._tensors[0])
+        check(self._grid._lib.dex_graph_launch(self._h))
+
+    def close(self):
+        if self._h.value:
+            h, self._h = self._h, C.c_void_p()
+            self._grid._graphs.remove(self)
+            check(self._grid._lib.dex_graph_destroy(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class HeliumGrid:
     """One complex-scaled two-electron sinc-DVR Hamiltonian on one GPU."""
 
@@ -102,9 +132,12 @@ class HeliumGrid:
         self.fast_path = bool(info.fast_path)
         self.device = device
         self._lib = lib
+        self._graphs: list = []  # MatvecGraph objects still open (freed before the context)
 
     # -- lifetime -----------------------------------------------------------------
     def close(self):
+        for gr in list(getattr(self, "_graphs", [])):
+            gr.close()
         if getattr(self, "_ctx", None) and self._ctx.value:
             self._lib.dex_ctx_destroy(self._ctx)
             self._ctx = C.c_void_p()
@@ -170,6 +203,22 @@ class HeliumGrid:
         check(self._lib.dex_matvec_sym_device(self._ctx, _dev(x), _dev(y), howmany, self.d,
                                               int(single), int(symmetry)))
         self.sync()
+
+    def matvec_graph(self, x, y, howmany: int | None = None, single: bool = False,
+                     symmetry: int = 0) -> "MatvecGraph":
+        """A HIP graph of the allocation-free matvec Y = H X on these device
+        tensors (dex_matvec_graph_create): ``launch()`` replays it on the grid's
+        stream, reading what ``x`` holds then (``sync()`` orders the host
+        after it); ``close()`` frees it (closing the grid frees it too)."""
+        howmany = x.numel() // self.d if howmany is None else howmany
+        _check_dev(x, howmany, self.d, self.device, single, "x")
+        _check_dev(y, howmany, self.d, self.device, single, "y")
+        h = C.c_void_p()
+        check(self._lib.dex_matvec_graph_create(self._ctx, _dev(x), _dev(y), howmany, self.d, int(single),
+                                                int(symmetry), C.byref(h)))
+        gr = MatvecGraph(self, h, (x, y))
+        self._graphs.append(gr)
+        return gr
 
     def dots_device(self, bras, kets, nn: int, mm: int) -> np.ndarray:
         out = np.zeros((nn, mm), dtype=np.complex128, order="F")

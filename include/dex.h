@@ -128,6 +128,23 @@ int dex_matvec_sym_device(dex_ctx *ctx, const void *in, void *out, int32_t howma
 int dex_matvec_noalloc(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
                        int64_t checksize, int32_t single, int32_t symmetry);
 
+/* A HIP graph of dex_matvec_noalloc(ctx, in, out, howmany, checksize, single,
+ * symmetry), captured on the context's stream: each dex_graph_launch replays
+ * the same kernels on the same pointers (reading what `in` holds at replay
+ * time) as one launch on that stream; dex_sync orders the host after it.  No
+ * counterpart in the reference (one matvec call per MYMATMUL); it is for
+ * callers that repeat a block matvec in a loop, such as scans of many small
+ * solves, and want one host launch per block.  Same limits and errors as
+ * dex_matvec_noalloc (the capture records its launches; nothing runs at
+ * creation).  Replays share that function's static scratch array: replays and
+ * other allocation-free calls on one GPU must be ordered on one stream.
+ * Destroy a graph before its context. */
+typedef struct dex_graph dex_graph;
+int dex_matvec_graph_create(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
+                            int64_t checksize, int32_t single, int32_t symmetry, dex_graph **out_graph);
+int dex_graph_launch(dex_graph *graph);
+int dex_graph_destroy(dex_graph *graph);
+
 /* MYDOTSUB(BRAS,KETS,SIZE,NN,MM,OUTDOTS), doublyexcited.F90:47-59:
  * OUT(NN,MM) = BRAS^H KETS, column-major.  Host or device pointers. */
 int dex_dots(dex_ctx *ctx, const double *bras, const double *kets, int64_t size,
