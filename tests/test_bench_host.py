@@ -50,3 +50,22 @@ def test_oracle_solve_sample_runs_the_requested_iterations():
     q.inner_block = 2  # 2-column groups: not what the oracle's inner solve restates
     with pytest.raises(ValueError):
         bench.oracle_solve_sample(q, iterations=1)
+
+
+def test_roofline_request_is_the_first_request_on_the_largest_grid():
+    """The request the roofline pass serves alone: among the stream's 20
+    requests, the first on the largest grid with the widest block, with the
+    FP64 form's plan when both forms share it, and the pass's own iteration
+    count (no GPU: plans use the library's footprint count)."""
+    from doublyexcited_amd.serve import poisson_stream
+
+    sv = bench.SERVED
+    reqs = poisson_stream(sv["rate"], 20, sv["seed"], precision="mixed", plan=True, cap=sv["cap"],
+                          plan_precision="fp64")
+    i, q = bench.roofline_request(20, "mixed", same_plan=True)
+    key = max((2 * r.n + 1, r.nev) for r in reqs)
+    assert (2 * q.n + 1, q.nev) == key
+    assert all((2 * r.n + 1, r.nev) < key for r in reqs[:i])
+    assert q.arrival == 0.0 and q.maxit == bench.SOLO_ITERATIONS and q.precision == "mixed"
+    assert (q.ns, q.inner_iter, q.inner_block, q.guard) == (reqs[i].ns, reqs[i].inner_iter,
+                                                            reqs[i].inner_block, reqs[i].guard)
