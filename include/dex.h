@@ -130,11 +130,11 @@ int dex_matvec_noalloc(dex_ctx *ctx, const void *in, void *out, int32_t howmany,
 
 /* A HIP graph of dex_matvec_noalloc(ctx, in, out, howmany, checksize, single,
  * symmetry), captured on the context's stream: each dex_graph_launch replays
- * the same kernels on the same pointers (reading what `in` holds at replay
- * time) as one launch on that stream; dex_sync orders the host after it.  No
- * counterpart in the reference (one matvec call per MYMATMUL); it is for
- * callers that repeat a block matvec in a loop, such as scans of many small
- * solves, and want one host launch per block.  Same limits and errors as
+ * the same kernels on the same device pointers (reading what `in` holds at
+ * replay time) as one launch on that stream; dex_sync orders the host after
+ * it.  No counterpart in the reference (one matvec call per MYMATMUL); it is
+ * for callers that repeat a block matvec in a loop, such as scans of many
+ * small solves, and want one host launch per block.  Same limits and errors as
  * dex_matvec_noalloc (the capture records its launches; nothing runs at
  * creation).  Replays share that function's static scratch array: replays and
  * other allocation-free calls on one GPU must be ordered on one stream.
