@@ -173,32 +173,37 @@ def test_c_oracle_matches_numpy_oracle():
             assert np.max(np.abs(y[:, j] - ref)) <= 1e-14 * np.max(np.abs(ref))
 
 
-def test_larger_grid_restricts_to_the_smaller_one():
-    """The NN=13 Hamiltonian restricted to the centred NN=11 sub-grid is the
-    NN=11 Hamiltonian (same spacing and angle): the sinc-DVR kinetic matrix
-    depends on i - j alone and the Coulomb table on r alone.  So for X padded
-    by SIXD_FIX (utils.F90:82-198), the inner entries of H13 SIXD_FIX(X) equal
-    H11 X.  The reference's own run shows the same: its NN=13 run reads the
-    NN=11 eigenvector through SIXD_FIX (Out.S.D2d0.T0.2d0.N6.nopetsc:64) and
-    prints its expectation value as -0.71315414287425 + 0.03522745948576 i
-    (:67-68), the NN=11 eigenvalue (Out.S.N5:147).  A Coulomb table shifted by
-    one point, or a padding that is not centred, breaks the equality."""
+@pytest.mark.parametrize("n_big", [6, 9])
+def test_larger_grid_restricts_to_the_smaller_one(n_big):
+    """The NN=13 and NN=19 Hamiltonians restricted to the centred NN=11
+    sub-grid are the NN=11 Hamiltonian (same spacing and angle): the sinc-DVR
+    kinetic matrix depends on i - j alone and the Coulomb table on r alone.  So
+    for X padded by SIXD_FIX (utils.F90:82-198), the inner entries of
+    H SIXD_FIX(X) equal H11 X.  The reference's own runs show the same: its
+    NN=13 and NN=19 runs read the NN=11 eigenvector through SIXD_FIX
+    (Out.S.D2d0.T0.2d0.N6.nopetsc:64, Out.S.D2d0.T0.2d0.N9.nopetsc:64) and
+    both print its expectation value as -0.71315414287425 + 0.03522745948576 i
+    (:67-68 of each), the NN=11 eigenvalue (Out.S.N5:147).  A Coulomb table
+    shifted by one point, a wrong spacing, or a padding that is not centred
+    breaks the equality."""
     from doublyexcited_amd.eigvecs import sixd_fix
     from oracle.c_oracle import CHammult
 
-    h11, h13 = Hamiltonian(5, 2.0, 0.2), Hamiltonian(6, 2.0, 0.2)
+    nn = 2 * n_big + 1
+    h11, hb = Hamiltonian(5, 2.0, 0.2), Hamiltonian(n_big, 2.0, 0.2)
     rng = np.random.default_rng(11)
     x = rng.standard_normal(h11.d) + 1j * rng.standard_normal(h11.d)
-    x13 = sixd_fix(x.reshape(1, -1), 5, 6).reshape(-1)
-    y13 = CHammult(h13, c_potential=True)(x13).reshape((13,) * 6, order="F")
+    xb = sixd_fix(x.reshape(1, -1), 5, n_big).reshape(-1)
+    yb = CHammult(hb, c_potential=True)(xb)
     y11 = CHammult(h11, c_potential=True)(x)
-    inner = y13[(slice(1, 12),) * 6].reshape(-1, order="F")
+    off = n_big - 5
+    inner = yb.reshape((nn,) * 6, order="F")[(slice(off, off + 11),) * 6].reshape(-1, order="F")
     assert np.max(np.abs(inner - y11)) <= 1e-14 * np.max(np.abs(y11))
     # the bilinear expectation value, as GET_EXPECTS forms it, is the same on both grids
-    # (up to the summation order of the two dot products over 1.8M and 4.8M entries)
+    # (up to the summation order of the two dot products)
     e11 = (x @ y11) / (x @ x)
-    e13 = (x13 @ y13.reshape(-1, order="F")) / (x13 @ x13)
-    assert abs(e13 - e11) <= 1e-10 * abs(e11)
+    eb = (xb @ yb) / (xb @ xb)
+    assert abs(eb - e11) <= 1e-10 * abs(e11)
 
 
 @pytest.mark.parametrize("n,delta,theta", [(1, 0.6, 0.3), (2, 0.45, 0.35), (3, 0.5, 0.3)])
