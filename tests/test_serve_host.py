@@ -129,3 +129,28 @@ def test_reserve_precision_sets_the_admitted_footprint():
     assert r.need() == footprint_bytes(10, 16, "mixed", 1, 4, 4, 0)
     r.reserve_precision = "fp64"
     assert r.need() == footprint_bytes(10, 16, "fp64", 1, 4, 4, 0) > footprint_bytes(10, 16, "mixed", 1, 4, 4, 0)
+
+
+def test_plans_fit_the_cap_and_widen_where_memory_allows():
+    """plan_solve over the stream's grids and blocks: every plan fits the cap
+    by the library's own footprint count; NS respects one launch of the block
+    kernels (NS (NEV + guard) <= 48, (3 + NS)(NEV + guard) <= 96); the mixed
+    form's smaller search blocks never plan narrower than FP64; and the
+    NN=21 block-16 FP64 plan is the one its footprint forces: NS 1, no guard
+    columns, the inner solve of order 4 on 4-column groups."""
+    from doublyexcited_amd.serve import plan_solve
+
+    cap = 240e9
+    for n in range(5, 11):
+        for nev in (4, 8, 16):
+            plans = {}
+            for prec in ("fp64", "mixed"):
+                p = plan_solve(n, nev, prec, cap)
+                nb = nev + p["guard"]
+                assert footprint_bytes(n, nev, prec, p["ns"], p["inner_iter"], p["inner_block"],
+                                       p["guard"]) <= cap, (n, nev, prec, p)
+                assert 1 <= p["ns"] <= 4 and p["ns"] * nb <= 48 and (3 + p["ns"]) * nb <= 96
+                plans[prec] = p
+            assert plans["mixed"]["ns"] >= plans["fp64"]["ns"] or plans["mixed"]["guard"] > plans["fp64"]["guard"]
+    p21 = plan_solve(10, 16, "fp64", cap)
+    assert p21 == dict(ns=1, inner_iter=4, inner_block=4, guard=0)
